@@ -1,0 +1,242 @@
+// Plain-C entry points of the host library for scripting (tests, bench.py):
+// graph generation and .g I/O, the schedule compiler, plan flattening and
+// the device drivers.  Errors return negative values; gs_last_error() holds
+// the message.
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "dataloader.h"
+#include "generator.h"
+#include "gpu_plan.h"
+#include "graph.h"
+#include "graphindor_amd.h"
+#include "motif_generator.h"
+#include "schedule_IEP.h"
+
+#define GS_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+thread_local std::string g_err;
+
+struct GsSchedule {
+    Schedule_IEP* s;
+    FlatPlan flat;
+    int flat_mode = -1;
+};
+
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    } catch (...) {
+        g_err = "unknown exception";
+        return -1;
+    }
+}
+}  // namespace
+
+GS_API const char* gs_last_error() { return g_err.c_str(); }
+
+// ---- graphs ---------------------------------------------------------------
+
+GS_API void* gs_graph_load(const char* path) {
+    DataLoader d;
+    Graph* g = nullptr;
+    if (!d.fast_load(g, path)) {
+        g_err = std::string("cannot load ") + path;
+        return nullptr;
+    }
+    return g;
+}
+
+GS_API int gs_graph_save(void* g, const char* path, int32_t max_isect) {
+    return DataLoader::dump(*(Graph*)g, path, (uint32_t)max_isect) ? 0 : -1;
+}
+
+GS_API void* gs_graph_from_csr(int32_t n, int64_t nnz, const int64_t* vertex, const int32_t* edge, int64_t tri_cnt) {
+    Graph* g = new Graph();
+    g->v_cnt = n;
+    g->e_cnt = nnz;
+    g->tri_cnt = tri_cnt;
+    g->vertex = new e_index_t[(size_t)n + 1];
+    g->edge = new v_index_t[nnz > 0 ? (size_t)nnz : 1];
+    std::memcpy(g->vertex, vertex, sizeof(int64_t) * ((size_t)n + 1));
+    if (nnz > 0) std::memcpy(g->edge, edge, sizeof(int32_t) * (size_t)nnz);
+    return g;
+}
+
+GS_API void* gs_graph_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, int permute) {
+    int32_t mi = 0;
+    Graph* g = generate_rmat(scale, edge_factor, a, b, c, seed, permute != 0, &mi);
+    return g;
+}
+
+GS_API void* gs_graph_er(int32_t n, double p, uint64_t seed) {
+    int32_t mi = 0;
+    return generate_erdos_renyi(n, p, seed, &mi);
+}
+
+GS_API void gs_graph_free(void* g) { delete (Graph*)g; }
+
+// out: v_cnt, e_cnt, tri_cnt, second-largest degree
+GS_API void gs_graph_info(void* gp, int64_t* out) {
+    Graph* g = (Graph*)gp;
+    out[0] = g->v_cnt;
+    out[1] = g->e_cnt;
+    out[2] = g->tri_cnt;
+    out[3] = second_largest_degree(*g);
+}
+
+GS_API void gs_graph_csr(void* gp, const int64_t** vertex, const int32_t** edge) {
+    Graph* g = (Graph*)gp;
+    *vertex = g->vertex;
+    *edge = g->edge;
+}
+
+GS_API void gs_graph_set_tri_cnt(void* gp, int64_t t) { ((Graph*)gp)->tri_cnt = t; }
+
+GS_API int64_t gs_graph_count_triangles(void* gp) { return count_triangles(*(Graph*)gp); }
+
+GS_API void gs_graph_reduce_for_clique(void* gp) { reduce_edges_for_clique(*(Graph*)gp); }
+
+GS_API void gs_set_max_intersection_size(int32_t m) { VertexSet::max_intersection_size = m; }
+
+// ---- schedules -------------------------------------------------------------
+
+GS_API void gs_set_restrict_validation(int mode) {
+    Schedule_IEP::set_restrict_validation(mode == 1 ? RestrictValidation::Reference : RestrictValidation::Exact);
+}
+
+GS_API void* gs_schedule_new(int n, const char* adj, int perf, int restricts, int iep, int32_t v_cnt, int64_t e_cnt,
+                             int64_t tri_cnt, int vertex_induced) {
+    void* out = nullptr;
+    guarded([&] {
+        Pattern p(n, adj);
+        bool ok = true;
+        Schedule_IEP* s = new Schedule_IEP(p, ok, perf, restricts, iep != 0, v_cnt, e_cnt, tri_cnt, vertex_induced != 0);
+        out = new GsSchedule{s, FlatPlan(), -1};
+        return 0;
+    });
+    return out;
+}
+
+// The two-argument form: order as given, no restrictions, no IEP.
+GS_API void* gs_schedule_new_plain(int n, const char* adj) {
+    void* out = nullptr;
+    guarded([&] {
+        Pattern p(n, adj);
+        out = new GsSchedule{new Schedule_IEP(p.get_adj_mat_ptr(), n), FlatPlan(), -1};
+        return 0;
+    });
+    return out;
+}
+
+GS_API void gs_schedule_free(void* h) {
+    GsSchedule* g = (GsSchedule*)h;
+    if (!g) return;
+    delete g->s;
+    delete g;
+}
+
+GS_API void gs_schedule_set_redundancy(void* h, int64_t r) { ((GsSchedule*)h)->s->set_in_exclusion_optimize_redundancy(r); }
+GS_API int64_t gs_schedule_redundancy(void* h) { return ((GsSchedule*)h)->s->get_in_exclusion_optimize_redundancy(); }
+GS_API int gs_schedule_multiplicity(void* h) { return ((GsSchedule*)h)->s->get_multiplicity(); }
+GS_API void gs_schedule_print(void* h) { ((GsSchedule*)h)->s->print_schedule(); fflush(stdout); }
+
+// Count on K_m with the CPU engine semantics (schedule-compiler helper).
+GS_API int64_t gs_schedule_count_complete(void* h, int m) { return ((GsSchedule*)h)->s->count_on_complete_graph(m); }
+
+GS_API int gs_restrictions_valid(int n, const char* adj, int npairs, const int32_t* pairs) {
+    Pattern p(n, adj);
+    std::vector<std::pair<int, int> > v;
+    for (int i = 0; i < npairs; ++i) v.push_back(std::make_pair(pairs[2 * i], pairs[2 * i + 1]));
+    return Schedule_IEP::restrictions_exactly_valid(p.get_adj_mat_ptr(), n, v) ? 1 : 0;
+}
+
+GS_API int gs_iep_head_multiplicity(int n, const char* adj, int npairs, const int32_t* pairs, int k) {
+    Pattern p(n, adj);
+    std::vector<std::pair<int, int> > v;
+    for (int i = 0; i < npairs; ++i) v.push_back(std::make_pair(pairs[2 * i], pairs[2 * i + 1]));
+    return Schedule_IEP::iep_head_multiplicity(p.get_adj_mat_ptr(), n, v, k);
+}
+
+// Same flat layout as the reference shim's dump (oracle/ref_shim.cpp).
+GS_API int gs_schedule_dump(void* h, int32_t* out, int cap) {
+    const Schedule_IEP& s = *((GsSchedule*)h)->s;
+    std::vector<int32_t> v;
+    int n = s.get_size();
+    int P = n * (n - 1) / 2 + 1;
+    int I = (int)s.in_exclusion_optimize_vertex_id.size();
+    int A = (int)s.in_exclusion_optimize_coef.size();
+    int head[] = {n, s.get_total_prefix_num(), s.get_basic_prefix_num(), s.get_total_restrict_num(),
+                  s.get_in_exclusion_optimize_num(), s.is_vertex_induced ? 1 : 0,
+                  (int)s.get_in_exclusion_optimize_redundancy(), I, A, (int)s.restrict_pair.size()};
+    v.insert(v.end(), head, head + 10);
+    v.insert(v.end(), s.get_adj_mat_ptr(), s.get_adj_mat_ptr() + n * n);
+    v.insert(v.end(), s.get_father_prefix_id_ptr(), s.get_father_prefix_id_ptr() + P);
+    v.insert(v.end(), s.get_last_ptr(), s.get_last_ptr() + n);
+    v.insert(v.end(), s.get_next_ptr(), s.get_next_ptr() + P);
+    v.insert(v.end(), s.get_break_size_ptr(), s.get_break_size_ptr() + P);
+    v.insert(v.end(), s.get_loop_set_prefix_id_ptr(), s.get_loop_set_prefix_id_ptr() + n);
+    v.insert(v.end(), s.get_restrict_last_ptr(), s.get_restrict_last_ptr() + n);
+    v.insert(v.end(), s.get_restrict_next_ptr(), s.get_restrict_next_ptr() + P);
+    v.insert(v.end(), s.get_restrict_index_ptr(), s.get_restrict_index_ptr() + P);
+    for (int i = 0; i < P; ++i) v.push_back(s.get_prefix_only_need_size(i) ? 1 : 0);
+    for (int i = 0; i < I; ++i) v.push_back(s.in_exclusion_optimize_vertex_id[i]);
+    for (int i = 0; i < I; ++i) v.push_back(s.in_exclusion_optimize_vertex_flag[i] ? 1 : 0);
+    for (int i = 0; i < I; ++i) v.push_back(s.in_exclusion_optimize_vertex_coef[i]);
+    for (int i = 0; i < A; ++i) v.push_back(s.in_exclusion_optimize_coef[i]);
+    for (int i = 0; i < A; ++i) v.push_back(s.in_exclusion_optimize_flag[i] ? 1 : 0);
+    for (int i = 0; i < A; ++i) v.push_back(s.in_exclusion_optimize_ans_pos[i]);
+    for (const auto& p : s.restrict_pair) {
+        v.push_back(p.first);
+        v.push_back(p.second);
+    }
+    if ((int)v.size() > cap) return -(int)v.size();
+    std::memcpy(out, v.data(), v.size() * sizeof(int32_t));
+    return (int)v.size();
+}
+
+// Flattened plan for `mode`; the pointers stay valid while the schedule lives.
+GS_API const gm_plan_desc* gs_schedule_plan_desc(void* h, int mode) {
+    GsSchedule* g = (GsSchedule*)h;
+    if (g->flat_mode != mode) {
+        flatten_schedule(*g->s, mode, g->flat);
+        g->flat_mode = mode;
+    }
+    return &g->flat.desc;
+}
+
+// ---- device drivers ----------------------------------------------------------
+
+// Count on the GPU; returns 0 and fills count (after /redundancy), raw sum
+// and kernel time, or -1.
+GS_API int gs_gpu_count(void* gp, void* h, int mode, uint64_t* count, uint64_t* raw, double* kernel_ms) {
+    return guarded([&] {
+        GpuRunInfo info;
+        Graph* g = (Graph*)gp;
+        const Schedule_IEP& s = *((GsSchedule*)h)->s;
+        *count = (mode == GM_MODE_CLIQUE) ? gpu_clique_matching(g, s, &info) : gpu_pattern_matching(g, s, &info);
+        if (raw) *raw = info.raw_sum;
+        if (kernel_ms) *kernel_ms = info.kernel_ms;
+        return 0;
+    });
+}
+
+// ---- motifs ------------------------------------------------------------------
+
+GS_API int gs_motifs(int k, char* out, int cap) {
+    MotifGenerator mg(k);
+    std::vector<Pattern> v = mg.generate();
+    int need = (int)v.size() * k * k;
+    if (need > cap) return -need;
+    int pos = 0;
+    for (const auto& p : v)
+        for (int i = 0; i < k * k; ++i) out[pos++] = p.get_adj_mat_ptr()[i] ? '1' : '0';
+    return (int)v.size();
+}

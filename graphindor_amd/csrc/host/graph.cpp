@@ -1,0 +1,187 @@
+// Graph and the device drivers (reference src/graph.cpp; gpu/gpu_graph.cu:22-51,
+// gpu/gpu_mc.cu:29-59, gpu/gpu_clique.cu:366-570).
+#include "graph.h"
+
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "generator.h"
+#include "gpu_plan.h"
+#include "graphindor_amd.h"
+#include "motif_generator.h"
+#include "timeinterval.h"
+
+int VertexSet::max_intersection_size = -1;
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc != GM_OK) throw std::runtime_error(std::string(what) + ": " + gm_last_error());
+}
+
+unsigned long long run_device(Graph* g, const Schedule_IEP& s, int mode, GpuRunInfo* info) {
+    FlatPlan fp;
+    flatten_schedule(s, mode, fp);
+    gm_plan* plan = nullptr;
+    check(gm_plan_create(&fp.desc, &plan), "gm_plan_create");
+    gm_graph* dg = g->device_graph();
+    uint64_t raw = 0;
+    gm_stats st{};
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = gm_count(dg, plan, nullptr, 0, nullptr, &raw, &st);
+    auto t1 = std::chrono::steady_clock::now();
+    gm_plan_destroy(plan);
+    check(rc, "gm_count");
+    if (info) {
+        info->raw_sum = raw;
+        info->kernel_ms = st.kernel_ms;
+        info->region_s = std::chrono::duration<double>(t1 - t0).count();
+        info->alg_bytes = st.alg_bytes;
+    }
+    // Same unsigned division as gpu/gpu_graph.cu:44.
+    return raw / (unsigned long long)s.get_in_exclusion_optimize_redundancy();
+}
+
+}  // namespace
+
+void flatten_schedule(const Schedule_IEP& s, int mode, FlatPlan& fp) {
+    const int n = s.get_size();
+    const int P = n * (n - 1) / 2 + 1;
+    fp.adj.assign(s.get_adj_mat_ptr(), s.get_adj_mat_ptr() + n * n);
+    fp.father.assign(s.get_father_prefix_id_ptr(), s.get_father_prefix_id_ptr() + P);
+    fp.last.assign(s.get_last_ptr(), s.get_last_ptr() + n);
+    fp.next.assign(s.get_next_ptr(), s.get_next_ptr() + P);
+    fp.brk.assign(s.get_break_size_ptr(), s.get_break_size_ptr() + P);
+    fp.loopset.assign(s.get_loop_set_prefix_id_ptr(), s.get_loop_set_prefix_id_ptr() + n);
+    fp.rlast.assign(s.get_restrict_last_ptr(), s.get_restrict_last_ptr() + n);
+    fp.rnext.assign(s.get_restrict_next_ptr(), s.get_restrict_next_ptr() + P);
+    fp.rindex.assign(s.get_restrict_index_ptr(), s.get_restrict_index_ptr() + P);
+    fp.only.resize(P);
+    for (int i = 0; i < P; ++i) fp.only[i] = s.get_prefix_only_need_size(i) ? 1 : 0;
+    const size_t I = s.in_exclusion_optimize_vertex_id.size();
+    const size_t A = s.in_exclusion_optimize_coef.size();
+    fp.iep_id.assign(s.in_exclusion_optimize_vertex_id.begin(), s.in_exclusion_optimize_vertex_id.end());
+    fp.iep_vflag.resize(I);
+    fp.iep_vcoef.resize(I);
+    for (size_t i = 0; i < I; ++i) {
+        fp.iep_vflag[i] = s.in_exclusion_optimize_vertex_flag[i] ? 1 : 0;
+        fp.iep_vcoef[i] = s.in_exclusion_optimize_vertex_coef[i];
+    }
+    fp.iep_coef.assign(s.in_exclusion_optimize_coef.begin(), s.in_exclusion_optimize_coef.end());
+    fp.iep_flag.resize(A);
+    fp.iep_pos.resize(A);
+    for (size_t i = 0; i < A; ++i) {
+        fp.iep_flag[i] = s.in_exclusion_optimize_flag[i] ? 1 : 0;
+        fp.iep_pos[i] = s.in_exclusion_optimize_ans_pos[i];
+    }
+    gm_plan_desc& d = fp.desc;
+    d.size = n;
+    d.total_prefix_num = s.get_total_prefix_num();
+    d.basic_prefix_num = s.get_basic_prefix_num();
+    d.total_restrict_num = s.get_total_restrict_num();
+    d.in_exclusion_optimize_num = s.get_in_exclusion_optimize_num();
+    d.is_vertex_induced = s.is_vertex_induced ? 1 : 0;
+    d.mode = mode;
+    d.adj_mat = fp.adj.data();
+    d.father_prefix_id = fp.father.data();
+    d.last = fp.last.data();
+    d.next = fp.next.data();
+    d.break_size = fp.brk.data();
+    d.loop_set_prefix_id = fp.loopset.data();
+    d.restrict_last = fp.rlast.data();
+    d.restrict_next = fp.rnext.data();
+    d.restrict_index = fp.rindex.data();
+    d.only_need_size = fp.only.data();
+    d.iep_vertex_num = (int32_t)I;
+    d.iep_vertex_id = fp.iep_id.data();
+    d.iep_vertex_flag = fp.iep_vflag.data();
+    d.iep_vertex_coef = fp.iep_vcoef.data();
+    d.iep_array_size = (int32_t)A;
+    d.iep_coef = fp.iep_coef.data();
+    d.iep_flag = fp.iep_flag.data();
+    d.iep_ans_pos = fp.iep_pos.data();
+}
+
+Graph::~Graph() {
+    invalidate_device();
+    delete[] edge;
+    delete[] vertex;
+    delete[] edge_from;
+}
+
+gm_graph* Graph::device_graph() {
+    if (!dev_) check(gm_graph_create(vertex, edge, v_cnt, e_cnt, VertexSet::max_intersection_size, &dev_), "gm_graph_create");
+    return dev_;
+}
+
+void Graph::invalidate_device() {
+    if (dev_) gm_graph_destroy(dev_);
+    dev_ = nullptr;
+}
+
+void Graph::build_reverse_edges() {
+    delete[] edge_from;
+    edge_from = new v_index_t[e_cnt > 0 ? e_cnt : 1];
+    for (v_index_t u = 0; u < v_cnt; ++u)
+        for (e_index_t e = vertex[u]; e < vertex[u + 1]; ++e) edge_from[e] = u;
+}
+
+long long Graph::pattern_matching(const Schedule_IEP& schedule, bool /*clique*/) {
+    return (long long)run_device(this, schedule, GM_MODE_PATTERN, nullptr);
+}
+
+long long Graph::triangle_counting() { return count_triangles(*this); }
+
+void Graph::motif_counting(int pattern_size) {
+    double total_counting_time = 0;
+    TimeInterval allTime, tmpTime;
+    MotifGenerator mg(pattern_size);
+    std::vector<Pattern> motifs = mg.generate();
+    for (size_t i = 0; i < motifs.size(); ++i) {
+        Pattern p = motifs[i];
+        printf("pattern = \n");
+        p.print();
+        printf("max intersection size %d\n", VertexSet::max_intersection_size);
+        bool is_pattern_valid;
+        Schedule_IEP schedule_iep(p, is_pattern_valid, 1, 1, true, v_cnt, e_cnt, tri_cnt);
+        if (!is_pattern_valid) {
+            printf("pattern is invalid!\n");
+            continue;
+        }
+        tmpTime.check();
+        long long ans = pattern_matching(schedule_iep);
+        total_counting_time += tmpTime.print("Pattern counting time: ");
+        printf("ans: %lld\n", ans);
+    }
+    printf("Counting time cost: %.6lf s\n", total_counting_time);
+    allTime.print("Total time cost");
+}
+
+// Keep N(v) ∩ [0, v) in place (src/graph.cpp:710-735).
+void reduce_edges_for_clique(Graph& g) {
+    printf("Trying to reduce edge. the pattern is a clique.\n");
+    printf("start erasing edge\n");
+    fflush(stdout);
+    e_index_t kept = 0;
+    for (v_index_t v = 0; v < g.v_cnt; ++v) {
+        e_index_t begin = kept;
+        for (e_index_t e = g.vertex[v]; e < g.vertex[v + 1] && e < g.e_cnt; ++e)
+            if (g.edge[e] < v) g.edge[kept++] = g.edge[e];
+        g.vertex[v] = begin;
+    }
+    g.vertex[g.v_cnt] = kept;
+    g.e_cnt = kept;
+    printf("newe: %ld\n", (long)g.e_cnt);
+    printf("Finish reduce.\n");
+    g.invalidate_device();
+}
+
+unsigned long long gpu_pattern_matching(Graph* g, const Schedule_IEP& s, GpuRunInfo* info) {
+    return run_device(g, s, GM_MODE_PATTERN, info);
+}
+
+unsigned long long gpu_clique_matching(Graph* g, const Schedule_IEP& s, GpuRunInfo* info) {
+    return run_device(g, s, GM_MODE_CLIQUE, info);
+}

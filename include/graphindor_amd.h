@@ -1,0 +1,150 @@
+/*
+ * graphindor_amd.h -- C ABI of the MI355X (gfx950) pattern-mining engine.
+ *
+ * This is the boundary between the host C++ library (Graph / Pattern /
+ * Schedule_IEP and the gpu_graph / gpu_clique / gpu_mc command lines) and
+ * the HIP kernels in libgm_hip.so.  Signatures carry plain pointers, sizes
+ * and POD structs only.
+ *
+ * What it replaces in the reference (sth1997/GraphIndor):
+ *   - the file-local device driver `pattern_matching(Graph*, const Schedule_IEP&)`
+ *     of gpu/gpu_graph.cu:22-51 and gpu/gpu_mc.cu:29-59, and
+ *     `pattern_matching_init` of gpu/gpu_clique.cu:366-570;
+ *   - `PatternMatchingDeviceContext::init/destroy`
+ *     (gpu/function/pattern_matching.cuh:18-104): CSR upload, scratch, counters;
+ *   - `GPUSchedule::create_from_schedule` (gpu/component/gpu_schedule.cuh:89-190):
+ *     the flattened schedule ("plan"), whose fields gm_plan_desc mirrors.
+ *
+ * Conventions: every int-returning entry point returns GM_OK (0) on success
+ * and a negative GM_ERR_* code otherwise; it never calls exit().  The message
+ * of the last failure on the calling thread is gm_last_error().  Handles are
+ * immutable after creation; one handle may be used from one host thread at a
+ * time.  The library owns all device memory; callers own their host buffers.
+ * The division by the IEP redundancy stays with the caller, as in the
+ * reference (gpu/gpu_graph.cu:44, src/graph.cpp:354).
+ */
+#ifndef GRAPHINDOR_AMD_H
+#define GRAPHINDOR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_OK 0
+#define GM_ERR_INVALID (-1)     /* bad argument or malformed plan/graph */
+#define GM_ERR_HIP (-2)         /* HIP runtime error (message says which) */
+#define GM_ERR_OOM (-3)         /* device allocation failed */
+#define GM_ERR_UNSUPPORTED (-4) /* pattern too large, no device, ... */
+
+/* Largest pattern the kernels enumerate; the reference GPU DFS stops at
+ * MAX_DEPTH = 7 (gpu/function/pattern_matching.cuh:7). */
+#define GM_MAX_PATTERN 7
+/* Prefix-table length for GM_MAX_PATTERN: n(n-1)/2 + 1
+ * (src/schedule_IEP.cpp:38-39). */
+#define GM_MAX_PREFIX (GM_MAX_PATTERN * (GM_MAX_PATTERN - 1) / 2 + 1)
+/* Upper bound on IEP table entries: Bell(5) groups x 5 blocks. */
+#define GM_MAX_IEP 64
+
+/* Counting semantics of a plan. */
+#define GM_MODE_PATTERN 0 /* gpu_graph / gpu_mc: restrictions, IEP, embedding subtraction
+                             (gpu/src/gpu_pattern_matching.cuh:14-104) */
+#define GM_MODE_CLIQUE 1  /* gpu_clique: graph already reduced to one direction per edge;
+                             the last level adds |loop set| (gpu/gpu_clique.cu:215-258) */
+
+typedef struct gm_graph gm_graph;
+typedef struct gm_plan gm_plan;
+
+/* Flattened schedule; field-for-field the GPUSchedule of
+ * gpu/component/gpu_schedule.cuh:41-84.  Arrays indexed by prefix id have
+ * n(n-1)/2+1 entries, arrays indexed by pattern vertex have n entries. */
+typedef struct gm_plan_desc {
+    int32_t size;                      /* pattern vertices n, 3 <= n <= GM_MAX_PATTERN */
+    int32_t total_prefix_num;
+    int32_t basic_prefix_num;
+    int32_t total_restrict_num;
+    int32_t in_exclusion_optimize_num; /* IEP depth k (0 = off) */
+    int32_t is_vertex_induced;
+    int32_t mode;                      /* GM_MODE_* */
+    const int32_t* adj_mat;            /* n*n row-major 0/1 */
+    const int32_t* father_prefix_id;   /* [P] -1 = root (aliases a neighbour list) */
+    const int32_t* last;               /* [n] head of prefixes built when vertex i is fixed */
+    const int32_t* next;               /* [P] next prefix in that list, -1 ends */
+    const int32_t* break_size;         /* [P] prune when |prefix| == break_size */
+    const int32_t* loop_set_prefix_id; /* [n] candidate set of vertex i */
+    const int32_t* restrict_last;      /* [n] head of restrictions checked at depth i */
+    const int32_t* restrict_next;      /* [P] */
+    const int32_t* restrict_index;     /* [P] emb[i] must be < emb[restrict_index] */
+    const uint8_t* only_need_size;     /* [P] size-only intersection */
+    int32_t iep_vertex_num;            /* I */
+    const int32_t* iep_vertex_id;      /* [I] prefix whose size feeds the IEP */
+    const uint8_t* iep_vertex_flag;    /* [I] 1: ans = |S| - coef; 0: ans = |S \ emb| */
+    const int32_t* iep_vertex_coef;    /* [I] */
+    int32_t iep_array_size;            /* A */
+    const int32_t* iep_coef;           /* [A] group coefficient on the closing entry */
+    const uint8_t* iep_flag;           /* [A] 1 closes a product group */
+    const int32_t* iep_ans_pos;        /* [A] index into ans[] */
+} gm_plan_desc;
+
+/* Launch tuning; zero fields mean "library default". */
+typedef struct gm_tuning {
+    int32_t waves_per_cu;   /* resident waves per CU of the persistent grid */
+    int32_t chunk_edges;    /* root edges claimed per atomic work-steal */
+    int32_t lds_set_cap;    /* max elements of a prefix set staged in LDS */
+    int32_t reserved[5];
+} gm_tuning;
+
+/* Per-call measurements, filled when the pointer is non-null. */
+typedef struct gm_stats {
+    double kernel_ms;       /* HIP-event time of the counting kernel(s) on the stream */
+    uint64_t alg_bytes;     /* algorithmic bytes (only when GM_COUNT_ALG_BYTES is set) */
+    uint64_t root_tasks;    /* root edges the grid claimed */
+    int32_t grid_waves;     /* persistent waves launched */
+    int32_t flags_used;
+} gm_stats;
+
+/* gm_count flags */
+#define GM_COUNT_ALG_BYTES 1 /* also tally the algorithmic byte model (slower) */
+
+/* Upload a CSR graph (row_ptr[n+1] int64, col[nnz] int32, rows sorted) to
+ * the current device.  max_isect bounds every materialised intersection
+ * (the .g header's max_intersection_size, src/dataloader.cpp:100; the
+ * reference sizes scratch from it, gpu/function/pattern_matching.cuh:25).
+ * Replaces the CSR part of PatternMatchingDeviceContext::init
+ * (gpu/function/pattern_matching.cuh:27-61). */
+int gm_graph_create(const int64_t* row_ptr, const int32_t* col, int32_t n, int64_t nnz,
+                    int32_t max_isect, gm_graph** out);
+void gm_graph_destroy(gm_graph* g);
+
+/* Copy a flattened schedule to the device.  Replaces
+ * GPUSchedule::create_from_schedule (gpu/component/gpu_schedule.cuh:89-190). */
+int gm_plan_create(const gm_plan_desc* desc, gm_plan** out);
+void gm_plan_destroy(gm_plan* p);
+
+/* Count embeddings: raw sum before the division by the IEP redundancy.
+ * Enqueues on `stream` (a hipStream_t, NULL = default stream) and
+ * synchronises it before returning.  Replaces the launch + sync + 8-byte
+ * readback of gpu/gpu_graph.cu:38-42 (gpu/gpu_clique.cu:522-529 for
+ * GM_MODE_CLIQUE). */
+int gm_count(const gm_graph* g, const gm_plan* p, const gm_tuning* tuning, int32_t flags,
+             void* stream, uint64_t* raw_sum, gm_stats* stats);
+
+/* Count several plans over one graph (e.g. every connected k-motif) in one
+ * persistent launch; raw_sums[np].  Replaces the per-motif init + launch
+ * loop of gpu/gpu_mc.cu:96-115. */
+int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np,
+                   const gm_tuning* tuning, int32_t flags, void* stream, uint64_t* raw_sums,
+                   gm_stats* stats);
+
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int gm_device_count(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHINDOR_AMD_H */

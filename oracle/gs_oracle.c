@@ -1,0 +1,491 @@
+/*
+ * gs_oracle.c -- CPU oracle for the pattern-mining hot path.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load this file's library (oracle/liboracle.so); the
+ * product never does.  It restates, in plain C with OpenMP, the counting
+ * rules the device kernels must reproduce bit for bit:
+ *
+ *   oracle_count_vertex_rooted  Graph::pattern_matching + pattern_matching_aggressive_func
+ *                               (src/graph.cpp:305-355, 429-566), VertexSet
+ *                               (src/vertex_set.cpp:83-159, 203-215, 256-277).
+ *   oracle_count_edge_rooted    the reference GPU driver's rule: root = a directed
+ *                               edge (gpu/src/gpu_pattern_matching.cuh:14-104), DFS of
+ *                               gpu/function/pattern_matching.cuh:110-288, set ops of
+ *                               gpu/component/gpu_vertex_set.cuh:67-248; it also
+ *                               tallies the algorithmic byte model (below).
+ *   oracle_count_clique         gpu_clique's rule (gpu/gpu_clique.cu:215-364).
+ *   oracle_brute_force          schedule-free: injective edge-preserving maps of the
+ *                               pattern (optionally also non-edge-preserving for
+ *                               vertex-induced) divided by |Aut|; pins everything above.
+ *
+ * Pinning (see tests/): wiki-vote triangles 608389 and 4-cliques 2077903
+ * (include/dataloader.h:32 and an independent brute force); closed forms on
+ * complete graphs; brute force on seeded small graphs; the four rules agree
+ * with each other; and, in this container, the reference's own
+ * Graph::pattern_matching built from source (oracle/_ref).
+ *
+ * Algorithmic byte model (per launch, summed over the enumeration):
+ *   root edge processed          8 (endpoints) + 16 (row pointers)
+ *   loop element consumed        4; +8 row pointers when it is expanded
+ *   materialised C = A ∩ B       4 (|A| + |B| + |C|)
+ *   size-only |A ∩ B|            4 (|A| + |B|)
+ *   subtraction / membership     4 |emb|
+ * Size-only prefixes are derived from the plan structure (no children, not a
+ * loop set before the IEP depth, every IEP row reading them has flag 1), the
+ * same rule the device uses.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#include "../include/graphindor_amd.h"
+
+typedef struct {
+    int32_t n;
+    int64_t nnz;
+    const int64_t* vertex;
+    const int32_t* edge;
+} or_graph;
+
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const int32_t* ptr;
+    int32_t len;
+} or_set;
+
+typedef struct {
+    const or_graph* g;
+    const gm_plan_desc* p;
+    int32_t* store;   /* P + n buffers of cap ints */
+    int64_t cap;
+    or_set* sets;     /* P + n (slot P+depth: vertex-induced scratch) */
+    uint32_t emb[GM_MAX_PATTERN + 1];
+    int emb_n;
+    int64_t ans[GM_MAX_IEP];
+    uint64_t sum;
+    uint64_t bytes;
+    uint8_t size_only[GM_MAX_PREFIX];
+    int gpu_rule;     /* 1: edge-rooted device rule, 0: CPU engine rule */
+} or_ctx;
+
+static int64_t max_degree(const or_graph* g) {
+    int64_t m = 0;
+    for (int32_t v = 0; v < g->n; ++v) {
+        int64_t d = g->vertex[v + 1] - g->vertex[v];
+        if (d > m) m = d;
+    }
+    return m;
+}
+
+/* Size-only rule shared with the device. */
+static void derive_size_only(const gm_plan_desc* p, uint8_t* out) {
+    int P = p->total_prefix_num;
+    int tail = p->size - p->in_exclusion_optimize_num;
+    for (int i = 0; i < P; ++i) out[i] = 0;
+    if (p->in_exclusion_optimize_num == 0) return;
+    for (int i = 0; i < p->iep_vertex_num; ++i) out[p->iep_vertex_id[i]] = 1;
+    for (int i = 0; i < p->iep_vertex_num; ++i)
+        if (!p->iep_vertex_flag[i]) out[p->iep_vertex_id[i]] = 0;
+    for (int q = 0; q < P; ++q)
+        if (p->father_prefix_id[q] >= 0) out[p->father_prefix_id[q]] = 0;
+    for (int d = 1; d < tail; ++d)
+        if (p->loop_set_prefix_id[d] >= 0) out[p->loop_set_prefix_id[d]] = 0;
+}
+
+static int in_emb(const or_ctx* c, uint32_t v) {
+    for (int i = 0; i < c->emb_n; ++i)
+        if (c->emb[i] == v) return 1;
+    return 0;
+}
+
+static int bsearch_u32(const int32_t* a, int32_t n, uint32_t x) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t y = (uint32_t)a[mid];
+        if (y == x) return 1;
+        if (y < x) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return 0;
+}
+
+/* First index with a[i] >= x (unsigned compare, as the device's lower_bound). */
+static int32_t lower_bound_u32(const int32_t* a, int32_t n, uint32_t x) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        if ((uint32_t)a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+/* |set[0:limit) \ emb| */
+static int32_t subtraction_size(or_ctx* c, const int32_t* set, int32_t limit) {
+    int32_t r = limit;
+    for (int i = 0; i < c->emb_n; ++i)
+        if (bsearch_u32(set, limit, c->emb[i])) --r;
+    c->bytes += 4ull * (uint64_t)c->emb_n;
+    return r;
+}
+
+static void build(or_ctx* c, int pid, const int32_t* in, int32_t n_in) {
+    int f = c->p->father_prefix_id[pid];
+    if (f == -1) {
+        c->sets[pid].ptr = in;
+        c->sets[pid].len = n_in;
+        return;
+    }
+    const int32_t* a = c->sets[f].ptr;
+    int32_t na = c->sets[f].len;
+    int32_t i = 0, j = 0, k = 0;
+    if (c->gpu_rule && c->size_only[pid]) {
+        while (i < na && j < n_in) {
+            if (a[i] < in[j]) ++i;
+            else if (a[i] > in[j]) ++j;
+            else { ++k; ++i; ++j; }
+        }
+        c->sets[pid].ptr = NULL;
+        c->sets[pid].len = k;
+        c->bytes += 4ull * (uint64_t)(na + n_in);
+        return;
+    }
+    int32_t* out = c->store + (int64_t)pid * c->cap;
+    while (i < na && j < n_in) {
+        if (a[i] < in[j]) ++i;
+        else if (a[i] > in[j]) ++j;
+        else { out[k++] = a[i]; ++i; ++j; }
+    }
+    c->sets[pid].ptr = out;
+    c->sets[pid].len = k;
+    c->bytes += 4ull * (uint64_t)(na + n_in + k);
+}
+
+static void iep_terminal(or_ctx* c) {
+    const gm_plan_desc* p = c->p;
+    for (int i = 0; i < p->iep_vertex_num; ++i) {
+        or_set s = c->sets[p->iep_vertex_id[i]];
+        if (p->iep_vertex_flag[i]) c->ans[i] = (int32_t)(s.len - p->iep_vertex_coef[i]);
+        else c->ans[i] = subtraction_size(c, s.ptr, s.len);
+    }
+    int last_pos = -1;
+    int64_t val = 0;
+    for (int pos = 0; pos < p->iep_array_size; ++pos) {
+        if (pos == last_pos + 1) val = c->ans[p->iep_ans_pos[pos]];
+        else if (val != 0) val = (int64_t)((uint64_t)val * (uint64_t)c->ans[p->iep_ans_pos[pos]]);
+        if (p->iep_flag[pos]) {
+            last_pos = pos;
+            c->sum += (uint64_t)val * (uint64_t)(int64_t)p->iep_coef[pos];
+        }
+    }
+}
+
+/* remove_anti_edge_vertices (src/graph.cpp:195-232,
+ * gpu/function/pattern_matching.cuh:145-206) into slot P + depth. */
+static or_set remove_anti_edge(or_ctx* c, or_set in, int depth) {
+    const gm_plan_desc* p = c->p;
+    int32_t* out = c->store + (int64_t)(p->total_prefix_num + depth) * c->cap;
+    int32_t k = 0;
+    for (int32_t i = 0; i < in.len; ++i) {
+        int keep = 1;
+        for (int u = 0; u < depth && keep; ++u) {
+            if (p->adj_mat[u * p->size + depth] != 0) continue;
+            uint32_t v = c->emb[u];
+            int64_t l = c->g->vertex[v], r = c->g->vertex[v + 1];
+            if (bsearch_u32(c->g->edge + l, (int32_t)(r - l), (uint32_t)in.ptr[i])) keep = 0;
+        }
+        if (keep) out[k++] = in.ptr[i];
+    }
+    or_set o = {out, k};
+    return o;
+}
+
+static uint32_t restriction_bound(const or_ctx* c, int depth, uint32_t init) {
+    uint32_t b = init;
+    for (int i = c->p->restrict_last[depth]; i != -1; i = c->p->restrict_next[i]) {
+        uint32_t e = c->emb[c->p->restrict_index[i]];
+        if (e < b) b = e;
+    }
+    return b;
+}
+
+static void dfs(or_ctx* c, int depth) {
+    const gm_plan_desc* p = c->p;
+    int n = p->size, k = p->in_exclusion_optimize_num;
+    or_set loop = c->sets[p->loop_set_prefix_id[depth]];
+    if (!c->gpu_rule) {
+        /* CPU engine: empty loop set returns before anything else. */
+        if (loop.len <= 0) return;
+        if (p->is_vertex_induced) loop = remove_anti_edge(c, loop, depth);
+        if (depth == n - k) { iep_terminal(c); return; }
+    } else {
+        if (depth == n - k) { iep_terminal(c); return; }
+        if (p->is_vertex_induced) loop = remove_anti_edge(c, loop, depth);
+    }
+    if (depth == n - 1 && k == 0) {
+        if (c->gpu_rule) {
+            uint32_t b = restriction_bound(c, depth, 0xffffffffu);
+            int32_t lim = lower_bound_u32(loop.ptr, loop.len, b);
+            c->sum += (uint64_t)(int64_t)subtraction_size(c, loop.ptr, lim);
+        } else if (p->total_restrict_num > 0) {
+            uint32_t b = restriction_bound(c, depth, (uint32_t)c->g->n);
+            int32_t lim = lower_bound_u32(loop.ptr, loop.len, b);
+            if (lim > 0) c->sum += (uint64_t)(int64_t)subtraction_size(c, loop.ptr, lim);
+        } else {
+            c->sum += (uint64_t)(int64_t)subtraction_size(c, loop.ptr, loop.len);
+        }
+        return;
+    }
+    uint32_t bound = restriction_bound(c, depth, c->gpu_rule ? 0xffffffffu : (uint32_t)c->g->n);
+    for (int32_t i = 0; i < loop.len; ++i) {
+        uint32_t v = (uint32_t)loop.ptr[i];
+        if (bound <= v) break;
+        c->bytes += 4ull;
+        if (in_emb(c, v)) continue;
+        int64_t l = c->g->vertex[v], r = c->g->vertex[v + 1];
+        c->bytes += 8ull;
+        int prune = 0;
+        for (int pid = p->last[depth]; pid != -1; pid = p->next[pid]) {
+            build(c, pid, c->g->edge + l, (int32_t)(r - l));
+            if (c->sets[pid].len == p->break_size[pid]) { prune = 1; break; }
+        }
+        if (prune) continue;
+        c->emb[c->emb_n++] = v;
+        dfs(c, depth + 1);
+        --c->emb_n;
+    }
+}
+
+static or_ctx* ctx_new(const or_graph* g, const gm_plan_desc* p, int gpu_rule) {
+    or_ctx* c = (or_ctx*)calloc(1, sizeof(or_ctx));
+    c->g = g;
+    c->p = p;
+    c->cap = max_degree(g) + 1;
+    c->store = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->cap * (size_t)(p->total_prefix_num + p->size));
+    c->sets = (or_set*)calloc((size_t)(p->total_prefix_num + p->size), sizeof(or_set));
+    c->gpu_rule = gpu_rule;
+    derive_size_only(p, c->size_only);
+    return c;
+}
+
+static void ctx_free(or_ctx* c) {
+    free(c->store);
+    free(c->sets);
+    free(c);
+}
+
+static int plan_ok(const gm_plan_desc* p) {
+    return p && p->size >= 2 && p->size <= GM_MAX_PATTERN && p->total_prefix_num <= GM_MAX_PREFIX &&
+           p->iep_vertex_num <= GM_MAX_IEP && p->iep_array_size <= GM_MAX_IEP;
+}
+
+/* Raw sum (before the redundancy division) with the CPU engine's rule. */
+uint64_t oracle_count_vertex_rooted(const or_graph* g, const gm_plan_desc* p, int threads) {
+    if (!plan_ok(p)) return 0;
+    uint64_t total = 0;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel reduction(+ : total)
+    {
+        or_ctx* c = ctx_new(g, p, 0);
+#pragma omp for schedule(dynamic)
+        for (int32_t v0 = 0; v0 < g->n; ++v0) {
+            const int32_t* nb = g->edge + g->vertex[v0];
+            int32_t d = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
+            for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, nb, d);
+            c->emb[0] = (uint32_t)v0;
+            c->emb_n = 1;
+            dfs(c, 1);
+        }
+        total += c->sum;
+        ctx_free(c);
+    }
+    return total;
+}
+
+/* Root edge index -> source vertex, for a contiguous range of edges. */
+static int32_t edge_source(const or_graph* g, int64_t e) {
+    int32_t lo = 0, hi = g->n - 1;
+    while (lo < hi) {
+        int32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (g->vertex[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* Raw sum with the device rule; *alg_bytes receives the byte model. */
+uint64_t oracle_count_edge_rooted(const or_graph* g, const gm_plan_desc* p, int threads, uint64_t* alg_bytes) {
+    if (!plan_ok(p)) return 0;
+    uint64_t total = 0, bytes = 0;
+    const int64_t chunk = 256;
+    int64_t nchunks = (g->nnz + chunk - 1) / chunk;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel reduction(+ : total, bytes)
+    {
+        or_ctx* c = ctx_new(g, p, 1);
+#pragma omp for schedule(dynamic)
+        for (int64_t ch = 0; ch < nchunks; ++ch) {
+            int64_t e0 = ch * chunk, e1 = e0 + chunk < g->nnz ? e0 + chunk : g->nnz;
+            int32_t v0 = edge_source(g, e0);
+            for (int64_t e = e0; e < e1; ++e) {
+                while (g->vertex[v0 + 1] <= e) ++v0;
+                uint32_t v1 = (uint32_t)g->edge[e];
+                if (p->restrict_last[1] != -1 && (uint32_t)v0 <= v1) continue;
+                c->bytes += 24ull;
+                const int32_t* n0 = g->edge + g->vertex[v0];
+                int32_t d0 = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
+                for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, n0, d0);
+                const int32_t* n1 = g->edge + g->vertex[v1];
+                int32_t d1 = (int32_t)(g->vertex[v1 + 1] - g->vertex[v1]);
+                int zero = 0;
+                for (int pid = p->last[1]; pid != -1; pid = p->next[pid]) {
+                    build(c, pid, n1, d1);
+                    if (c->sets[pid].len == 0 && pid < p->basic_prefix_num) { zero = 1; break; }
+                }
+                if (zero) continue;
+                c->emb[0] = (uint32_t)v0;
+                c->emb[1] = v1;
+                c->emb_n = 2;
+                if (p->mode == GM_MODE_CLIQUE) {
+                    extern void oracle_clique_dfs(or_ctx*, int);
+                    oracle_clique_dfs(c, 2);
+                } else {
+                    dfs(c, 2);
+                }
+            }
+        }
+        total += c->sum;
+        bytes += c->bytes;
+        ctx_free(c);
+    }
+    if (alg_bytes) *alg_bytes = bytes;
+    return total;
+}
+
+/* gpu_clique's DFS: no restrictions, no membership test, the last level
+ * adds |loop set| (gpu/gpu_clique.cu:215-258). */
+void oracle_clique_dfs(or_ctx* c, int depth) {
+    const gm_plan_desc* p = c->p;
+    or_set loop = c->sets[p->loop_set_prefix_id[depth]];
+    if (depth == p->size - 1) {
+        c->sum += (uint64_t)loop.len;
+        return;
+    }
+    for (int32_t i = 0; i < loop.len; ++i) {
+        uint32_t v = (uint32_t)loop.ptr[i];
+        int64_t l = c->g->vertex[v], r = c->g->vertex[v + 1];
+        c->bytes += 12ull;
+        int prune = 0;
+        for (int pid = p->last[depth]; pid != -1; pid = p->next[pid]) {
+            build(c, pid, c->g->edge + l, (int32_t)(r - l));
+            if (c->sets[pid].len == p->break_size[pid]) { prune = 1; break; }
+        }
+        if (prune) continue;
+        c->emb[c->emb_n++] = v;
+        oracle_clique_dfs(c, depth + 1);
+        --c->emb_n;
+    }
+}
+
+uint64_t oracle_count_clique(const or_graph* g, const gm_plan_desc* p, int threads, uint64_t* alg_bytes) {
+    gm_plan_desc q = *p;
+    q.mode = GM_MODE_CLIQUE;
+    return oracle_count_edge_rooted(g, &q, threads, alg_bytes);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Schedule-free brute force. */
+
+typedef struct {
+    const or_graph* g;
+    int n;
+    const int32_t* adj;
+    int induced;
+    uint32_t m[GM_MAX_PATTERN + 1];
+    uint64_t count;
+} bf_ctx;
+
+static int has_edge(const or_graph* g, uint32_t a, uint32_t b) {
+    int64_t l = g->vertex[a], r = g->vertex[a + 1];
+    return bsearch_u32(g->edge + l, (int32_t)(r - l), b);
+}
+
+static void bf_rec(bf_ctx* c, int d) {
+    if (d == c->n) {
+        ++c->count;
+        return;
+    }
+    if (d >= GM_MAX_PATTERN) return; /* unreachable: n <= GM_MAX_PATTERN */
+    int anchor = -1;
+    for (int j = 0; j < d; ++j)
+        if (c->adj[j * c->n + d]) { anchor = j; break; }
+    const int32_t* cand;
+    int64_t nc;
+    if (anchor >= 0) {
+        uint32_t a = c->m[anchor];
+        cand = c->g->edge + c->g->vertex[a];
+        nc = c->g->vertex[a + 1] - c->g->vertex[a];
+    } else {
+        cand = NULL;
+        nc = c->g->n;
+    }
+    for (int64_t i = 0; i < nc; ++i) {
+        uint32_t v = cand ? (uint32_t)cand[i] : (uint32_t)i;
+        int ok = 1;
+        for (int j = 0; j < d && ok; ++j) {
+            if (c->m[j] == v) ok = 0;
+            else if (c->adj[j * c->n + d]) ok = has_edge(c->g, c->m[j], v);
+            else if (c->induced) ok = !has_edge(c->g, c->m[j], v);
+        }
+        if (!ok) continue;
+        c->m[d] = v;
+        bf_rec(c, d + 1);
+    }
+}
+
+static uint64_t automorphisms(int n, const int32_t* adj) {
+    int perm[GM_MAX_PATTERN];
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    uint64_t count = 0;
+    for (;;) {
+        int same = 1;
+        for (int i = 0; i < n && same; ++i)
+            for (int j = 0; j < n; ++j)
+                if ((adj[i * n + j] != 0) != (adj[perm[i] * n + perm[j]] != 0)) { same = 0; break; }
+        count += (uint64_t)same;
+        /* next permutation */
+        int i = n - 2;
+        while (i >= 0 && perm[i] >= perm[i + 1]) --i;
+        if (i < 0) break;
+        int j = n - 1;
+        while (perm[j] <= perm[i]) --j;
+        int t = perm[i]; perm[i] = perm[j]; perm[j] = t;
+        for (int a = i + 1, b = n - 1; a < b; ++a, --b) { t = perm[a]; perm[a] = perm[b]; perm[b] = t; }
+    }
+    return count;
+}
+
+/* Occurrences of the pattern: injective maps preserving its edges (and, if
+ * `induced`, its non-edges) divided by |Aut|. */
+uint64_t oracle_brute_force(const or_graph* g, int n, const int32_t* adj, int induced, int threads) {
+    if (n < 1 || n > GM_MAX_PATTERN) return 0;
+    uint64_t total = 0;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic) reduction(+ : total)
+    for (int32_t v0 = 0; v0 < g->n; ++v0) {
+        bf_ctx c;
+        memset(&c, 0, sizeof(c));
+        c.g = g;
+        c.n = n;
+        c.adj = adj;
+        c.induced = induced;
+        c.m[0] = (uint32_t)v0;
+        bf_rec(&c, 1);
+        total += c.count;
+    }
+    return total / automorphisms(n, adj);
+}

@@ -1,0 +1,206 @@
+"""The schedule compiler (host Schedule_IEP) against the reference's own
+schedules and against what the mathematics fixes.
+
+Reference-compatible mode must reproduce the reference's Schedule_IEP field
+for field (tests/golden/reference_schedules.json, written by
+tests/golden/make_fixtures.py from the reference built from source).  The
+default (exact) mode must give the same order and restrictions whenever the
+reference's restriction set is a valid symmetry breaking, and valid sets
+always.
+"""
+import json
+import math
+import os
+import random
+
+import pytest
+
+from graphindor_amd import native as N
+from oracle import oracle as O
+from tests.common import MOTIFS4, PATTERNS, clique_adj
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_schedules.json")
+
+
+def load_golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def split(d):
+    """Named fields of a schedule dump (layout of gs_schedule_dump / gsref_schedule_dump)."""
+    n = d[0]
+    P = n * (n - 1) // 2 + 1
+    I, A, R = d[7], d[8], d[9]
+    out = {"header": d[:10]}
+    off = 10
+    for name, L in [("adj", n * n), ("father", P), ("last", n), ("next", P), ("break_size", P), ("loopset", n),
+                    ("rlast", n), ("rnext", P), ("rindex", P), ("only", P), ("iep_id", I), ("iep_vflag", I),
+                    ("iep_vcoef", I), ("iep_coef", A), ("iep_flag", A), ("iep_pos", A), ("pairs", 2 * R)]:
+        out[name] = d[off:off + L]
+        off += L
+    T = d[3]
+    out["rindex"] = out["rindex"][:T]      # the reference leaves the rest uninitialised
+    for k in ("father", "next", "break_size", "rnext"):
+        out[k] = out[k][:P - 1]            # the reference never resets the last entry
+    return out
+
+
+@pytest.fixture
+def reference_mode():
+    N.set_restrict_validation("reference")
+    yield
+    N.set_restrict_validation("exact")
+
+
+def test_reference_mode_matches_golden(reference_mode):
+    g = load_golden()
+    for rec in g["schedules"]:
+        s = N.Schedule.build(rec["n"], rec["adj"], rec["v_cnt"], rec["e_cnt"], rec["tri_cnt"], iep=rec["iep"])
+        assert split(s.dump()) == split(rec["dump"]), (rec["graph"], rec["pattern"], rec["iep"])
+
+
+def test_exact_mode_keeps_valid_reference_choices():
+    g = load_golden()
+    differ = set()
+    for rec in g["schedules"]:
+        ref = split(rec["dump"])
+        pairs = list(zip(ref["pairs"][0::2], ref["pairs"][1::2]))
+        order_adj = "".join(str(x) for x in ref["adj"])
+        k = ref["header"][4]
+        valid = N.restrictions_valid(rec["n"], order_adj, pairs) and (
+            k < 2 or N.iep_head_multiplicity(rec["n"], order_adj, pairs, k) > 0)
+        s = N.Schedule.build(rec["n"], rec["adj"], rec["v_cnt"], rec["e_cnt"], rec["tri_cnt"], iep=rec["iep"])
+        mine = split(s.dump())
+        mp = list(zip(mine["pairs"][0::2], mine["pairs"][1::2]))
+        madj = "".join(str(x) for x in mine["adj"])
+        assert N.restrictions_valid(rec["n"], madj, mp)
+        mk = mine["header"][4]
+        assert mk < 2 or N.iep_head_multiplicity(rec["n"], madj, mp, mk) == mine["header"][6]
+        if valid:
+            for key in ("adj", "pairs", "loopset", "father", "last", "next", "break_size"):
+                assert mine[key] == ref[key], (rec["graph"], rec["pattern"], key)
+        else:
+            differ.add(rec["pattern"])
+    # The benchmark patterns whose reference restriction set is invalid.
+    assert "c5" in differ and differ <= {"c5", "prism", "c6"}, differ
+
+
+def test_reference_five_cycle_restrictions_are_invalid():
+    """The reference keeps a 5-cycle restriction set that drops embeddings
+    (its validation counts vertex-induced copies in K_6, which are 0 for any
+    non-clique, src/schedule_IEP.cpp:1737-1760)."""
+    g = load_golden()
+    rec = next(r for r in g["schedules"] if r["pattern"] == "c5" and r["graph"] == "rmat22_s4" and r["iep"])
+    ref = split(rec["dump"])
+    pairs = list(zip(ref["pairs"][0::2], ref["pairs"][1::2]))
+    assert not N.restrictions_valid(5, "".join(str(x) for x in ref["adj"]), pairs)
+
+
+def test_iep_coefficients_are_moebius_values():
+    """Each IEP group coefficient is the product over its blocks of
+    (-1)^(b-1) (b-1)! (Moebius function of the partition lattice)."""
+    for name in ("house", "c6", "star3", "path4"):
+        n, adj = PATTERNS[name]
+        s = N.Schedule.build(n, adj, 10000, 100000, 1000)
+        d = split(s.dump())
+        k = d["header"][4]
+        if k == 0:
+            continue
+        # Groups: consecutive entries ending with flag 1; block sizes are not
+        # dumped, but the closing coefficients must be a multiset of Moebius
+        # products over the set partitions of k elements.
+        want = sorted(moebius_products(k))
+        got = sorted(c for c, f in zip(d["iep_coef"], d["iep_flag"]) if f)
+        assert got == want, name
+
+
+def moebius_products(k):
+    def partitions(items):
+        if not items:
+            yield []
+            return
+        first, rest = items[0], items[1:]
+        for p in partitions(rest):
+            yield [[first]] + p
+            for i in range(len(p)):
+                yield p[:i] + [[first] + p[i]] + p[i + 1:]
+    out = []
+    for p in partitions(list(range(k))):
+        v = 1
+        for b in p:
+            v *= (-1) ** (len(b) - 1) * math.factorial(len(b) - 1)
+        out.append(v)
+    return out
+
+
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "c6", "prism", "diamond", "star3", "path4"])
+def test_redundancy_and_complete_graph_counts(name):
+    """On K_m the scheduled count divided by the redundancy equals the closed form."""
+    n, adj = PATTERNS[name]
+    s = N.Schedule.build(n, adj, 10 ** 6, 3 * 10 ** 7, 10 ** 8)
+    for m in (n, n + 1, 8):
+        want = math.perm(m, n) // __import__("tests.common", fromlist=["automorphisms"]).automorphisms(n, adj)
+        assert s.count_complete(m) == want
+
+
+def test_motif_generator_order_and_counts():
+    assert N.motifs(4) == MOTIFS4
+    assert [len(N.motifs(k)) for k in (3, 4, 5, 6)] == [2, 6, 21, 112]
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (reference sources absent)")
+def test_motif_generator_matches_reference():
+    import ctypes
+    for k in (3, 4, 5):
+        buf = ctypes.create_string_buffer(k * k * 200)
+        m = O.ref().gsref_motifs(k, buf, len(buf))
+        raw = buf.raw[:m * k * k].decode()
+        assert [raw[i * k * k:(i + 1) * k * k] for i in range(m)] == N.motifs(k)
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (reference sources absent)")
+def test_reference_mode_matches_live_reference(reference_mode):
+    """Random connected 4-6 vertex patterns at several graph statistics.
+    Skips the patterns on which the reference reads past its own IEP answer
+    buffer (stale positions; its redundancy is then undefined)."""
+    rng = random.Random(5)
+    stats = [(7115, 201524, 608389), (1 << 20, 31403774, 424298210), (16384, 425200, 2797282)]
+    checked = 0
+    for n in (4, 5, 6):
+        for _ in range(12):
+            adj = random_connected(n, rng)
+            for (v, e, t) in stats:
+                ref = split(O.ref_schedule_dump(n, adj, v, e, t))
+                if ref["iep_pos"] and max(ref["iep_pos"]) >= ref["header"][7]:
+                    continue
+                s = N.Schedule.build(n, adj, v, e, t)
+                assert split(s.dump()) == ref, (n, adj, v)
+                checked += 1
+    assert checked > 80
+
+
+def random_connected(n, rng):
+    while True:
+        m = [["0"] * n for _ in range(n)]
+        for i in range(n):
+            for j in range(i + 1, n):
+                if rng.random() < 0.5:
+                    m[i][j] = m[j][i] = "1"
+        seen, st = {0}, [0]
+        while st:
+            x = st.pop()
+            for y in range(n):
+                if m[x][y] == "1" and y not in seen:
+                    seen.add(y)
+                    st.append(y)
+        if len(seen) == n:
+            return "".join("".join(r) for r in m)
+
+
+def test_clique_schedules_are_plain_chains():
+    for k in range(3, 8):
+        s = N.Schedule.plain(k, clique_adj(k))
+        d = split(s.dump())
+        assert d["header"][4] == 0 and d["header"][3] == 0  # no IEP, no restrictions
+        assert d["loopset"] == [-1] + list(range(k - 1))
