@@ -1,0 +1,171 @@
+"""Benchmark: k-clique counting time on one MI355X (exact).
+
+Workload (one GPU): R-MAT 2^20 vertices, edge factor 16, (a,b,c,d) =
+(0.57,0.19,0.19,0.05), seed 2, symmetrised, cleaned, ids permuted; reduced
+to one direction per edge and counted with the gpu_clique rule
+(reference gpu/gpu_clique.cu).  `value` is the 5-clique counting time
+(kernel + sync + 8-byte readback, graph resident in HBM); the 4-clique time
+and an Erdos-Renyi G(2^20, 32/2^20) control (seed 3) are reported beside it.
+
+A step is one full count.  The first warm-up step also tallies the
+algorithmic byte model (4 B per list element read or written, per set
+operation), from which `roofline.achieved` = bytes / average kernel time.
+
+cpu_baseline: the oracle's restatement of the reference CPU engine
+(Graph::pattern_matching, vertex-rooted, OpenMP dynamic) with the same
+schedule the reference's clique_test builds, timed on a strided sample of
+root vertices (ids are randomly permuted, so the sample is unbiased) and
+scaled by the stride.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def clique_adj(k):
+    return "".join("0" if i == j else "1" for i in range(k) for j in range(k))
+
+
+def time_counts(N, dg, plan, steps, warmup):
+    """Warm-up (first one with the byte tally), then `steps` timed counts."""
+    bytes_ = None
+    raw = None
+    for i in range(max(1, warmup)):
+        r, st = N.count(dg, plan, flags=N.GM_COUNT_ALG_BYTES if i == 0 else 0)
+        if i == 0:
+            bytes_ = st.alg_bytes
+        raw = r if raw is None else raw
+        assert r == raw
+    walls, kms = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        r, st = N.count(dg, plan)
+        walls.append(time.perf_counter() - t0)
+        kms.append(st.kernel_ms)
+        assert r == raw, "count changed between steps"
+    return raw, bytes_, sum(walls) / len(walls), sum(kms) / len(kms)
+
+
+def cpu_baseline(N, O, v, e, k, target_s=15.0):
+    """Reference-CPU-engine restatement on a strided sample of roots."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    csr = O.CSR(v, e)
+    sched = N.Schedule.build(k, clique_adj(k), len(v) - 1, len(e), 0)
+    desc = sched.plan_desc()
+    stride = 4096
+    t0 = time.perf_counter()
+    O.count_vertex_rooted_strided(csr, desc, threads, stride, 0)
+    t = time.perf_counter() - t0
+    # Aim the sample at ~target_s of CPU work.
+    if t > 0:
+        stride = max(1, int(stride * t / target_s))
+    stride = min(stride, 4096)
+    t0 = time.perf_counter()
+    O.count_vertex_rooted_strided(csr, desc, threads, stride, 1 % stride)
+    t = time.perf_counter() - t0
+    return {"value": t * stride, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"{k}-clique, roots v0 = 1 mod {stride} of the same reduced graph "
+                      f"({t:.1f} s measured, x{stride})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-control", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the counting path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from graphindor_amd import native as N
+    if N.device_count() < 1:
+        sys.exit("no GPU visible")
+
+    t0 = time.time()
+    g = N.Graph.rmat(args.scale, 16, 2)
+    info = g.info()
+    g.reduce_for_clique()
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    log(f"graph: R-MAT 2^{args.scale} ef16 seed 2: {info['v_cnt']} vertices, {info['e_cnt']} CSR entries, "
+        f"{len(e)} after reduction ({time.time() - t0:.1f} s)")
+    dg = N.DeviceGraph(v, e)
+
+    results = {}
+    for k in (4, 5):
+        s = N.Schedule.plain(k, clique_adj(k))
+        s.redundancy = 1
+        plan = N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE))
+        steps = args.steps if k == 5 else 1
+        raw, nbytes, wall, kms = time_counts(N, dg, plan, steps, args.warmup if k == 5 else 1)
+        results[k] = dict(count=raw, alg_bytes=nbytes, wall_s=wall, kernel_ms=kms)
+        log(f"{k}-clique: {raw} in {wall:.4f} s (kernel {kms:.2f} ms), alg bytes {nbytes}")
+
+    control = None
+    if not args.no_control:
+        ge = N.Graph.erdos_renyi(1 << args.scale, 32.0 / (1 << args.scale), 3)
+        ge.reduce_for_clique()
+        ve, ee = ge.csr()
+        dge = N.DeviceGraph(ve.copy(), ee.copy())
+        control = {}
+        for k in (4, 5):
+            s = N.Schedule.plain(k, clique_adj(k))
+            s.redundancy = 1
+            plan = N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE))
+            raw, nbytes, wall, kms = time_counts(N, dge, plan, 1, 1)
+            control[f"k{k}"] = {"count": raw, "time_s": round(wall, 6)}
+            log(f"ER control {k}-clique: {raw} in {wall:.4f} s")
+
+    r5 = results[5]
+    achieved = r5["alg_bytes"] / (r5["kernel_ms"] / 1e3) / 1e9
+    out = {
+        "metric": "5-vertex PM & 5-clique time on 1 MI355X (exact); learned vs plain schedule",
+        "value": round(r5["wall_s"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r5["wall_s"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64",
+        "data": "synthetic (seeded R-MAT / Erdos-Renyi, generated in-process)",
+        "config": {"workload": f"5-clique count, gpu_clique rule, R-MAT 2^{args.scale} ef16 seed 2 "
+                               "(a,b,c,d)=(0.57,0.19,0.19,0.05), plain schedule",
+                   "vertices": info["v_cnt"], "csr_entries": info["e_cnt"], "reduced_edges": int(len(e)),
+                   "count_5clique": r5["count"], "count_4clique": results[4]["count"],
+                   "time_4clique_s": round(results[4]["wall_s"], 6),
+                   "er_control": control},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "enumerate_kernel<GM_MODE_CLIQUE>", "kernel_ms": round(r5["kernel_ms"], 3),
+                     "alg_bytes": r5["alg_bytes"]},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        out["cpu_baseline"] = cpu_baseline(N, O, v, e, 5)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

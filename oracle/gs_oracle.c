@@ -283,16 +283,20 @@ static int plan_ok(const gm_plan_desc* p) {
            p->iep_vertex_num <= GM_MAX_IEP && p->iep_array_size <= GM_MAX_IEP;
 }
 
-/* Raw sum (before the redundancy division) with the CPU engine's rule. */
-uint64_t oracle_count_vertex_rooted(const or_graph* g, const gm_plan_desc* p, int threads) {
-    if (!plan_ok(p)) return 0;
+/* Raw sum (before the redundancy division) with the CPU engine's rule over
+ * the root vertices v0 = offset, offset + stride, ...  (stride 1: all). */
+uint64_t oracle_count_vertex_rooted_strided(const or_graph* g, const gm_plan_desc* p, int threads, int32_t stride,
+                                            int32_t offset) {
+    if (!plan_ok(p) || stride < 1) return 0;
     uint64_t total = 0;
+    int32_t nroots = offset < g->n ? (g->n - offset + stride - 1) / stride : 0;
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel reduction(+ : total)
     {
         or_ctx* c = ctx_new(g, p, 0);
 #pragma omp for schedule(dynamic)
-        for (int32_t v0 = 0; v0 < g->n; ++v0) {
+        for (int32_t r = 0; r < nroots; ++r) {
+            int32_t v0 = offset + r * stride;
             const int32_t* nb = g->edge + g->vertex[v0];
             int32_t d = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
             for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, nb, d);
@@ -304,6 +308,10 @@ uint64_t oracle_count_vertex_rooted(const or_graph* g, const gm_plan_desc* p, in
         ctx_free(c);
     }
     return total;
+}
+
+uint64_t oracle_count_vertex_rooted(const or_graph* g, const gm_plan_desc* p, int threads) {
+    return oracle_count_vertex_rooted_strided(g, p, threads, 1, 0);
 }
 
 /* Root edge index -> source vertex, for a contiguous range of edges. */

@@ -31,6 +31,9 @@ def lib():
         for name in ("oracle_count_vertex_rooted",):
             getattr(L, name).argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int]
             getattr(L, name).restype = ctypes.c_uint64
+        L.oracle_count_vertex_rooted_strided.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
+                                                         ctypes.c_int32, ctypes.c_int32]
+        L.oracle_count_vertex_rooted_strided.restype = ctypes.c_uint64
         for name in ("oracle_count_edge_rooted", "oracle_count_clique"):
             getattr(L, name).argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]
@@ -84,6 +87,11 @@ class CSR:
 
 def count_vertex_rooted(csr, desc_ptr, threads=0):
     return lib().oracle_count_vertex_rooted(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p), threads)
+
+
+def count_vertex_rooted_strided(csr, desc_ptr, threads, stride, offset=0):
+    return lib().oracle_count_vertex_rooted_strided(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p),
+                                                    threads, stride, offset)
 
 
 def count_edge_rooted(csr, desc_ptr, threads=0):
