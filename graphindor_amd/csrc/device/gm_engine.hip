@@ -43,6 +43,8 @@ struct DevPlan {
     int32_t father[GM_MAX_PREFIX], last[GM_MAX_PATTERN], next[GM_MAX_PREFIX], brk[GM_MAX_PREFIX];
     int32_t loopset[GM_MAX_PATTERN], rlast[GM_MAX_PATTERN], rnext[GM_MAX_PREFIX], rindex[GM_MAX_PREFIX];
     int32_t size_only[GM_MAX_PREFIX];
+    int32_t clique_tail;  // clique mode: depth n-2 counts lane-parallel (see clique_tail_count)
+    int32_t clique_chain; // clique mode on the plain k-clique chain: local-bitmap engine eligible
     int32_t iep_id[GM_MAX_IEP], iep_vflag[GM_MAX_IEP], iep_vcoef[GM_MAX_IEP];
     int32_t iep_coef[GM_MAX_IEP], iep_flag[GM_MAX_IEP], iep_pos[GM_MAX_IEP];
 };
@@ -53,9 +55,18 @@ struct SetRef {
     int32_t pad;
 };
 
+// Elements of a clique-mode tail loop set staged in LDS per wave.
+constexpr int kStage = 2048;
+
 struct WaveLds {
     SetRef sets[kMaxSets];
     int32_t ans[GM_MAX_IEP];
+};
+
+template <int MODE>
+struct WaveShared {
+    WaveLds w;
+    uint32_t stage[MODE == GM_MODE_CLIQUE ? kStage : 1];
 };
 
 struct Counters {
@@ -80,6 +91,7 @@ struct KernelArgs {
     int64_t nchunks;
     Counters* ctr;
     int32_t stats;
+    int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -138,10 +150,13 @@ struct Wave {
     KernelArgs a;
     const DevPlan* pl;
     WaveLds* lds;
+    uint32_t* stage;
     uint32_t* scratch;
     uint32_t emb[GM_MAX_PATTERN + 1];
-    unsigned long long sum;
-    unsigned long long bytes;
+    unsigned long long sum;        // wave-uniform
+    unsigned long long bytes;      // wave-uniform
+    unsigned long long lane_sum;   // per lane (lane-parallel paths)
+    unsigned long long lane_bytes; // per lane
     unsigned int overflow;
 };
 
@@ -323,6 +338,49 @@ __device__ void iep_terminal(Wave& w) {
     }
 }
 
+// Branchless membership test in an LDS-resident sorted array.
+__device__ __forceinline__ bool lds_contains(const uint32_t* b, int32_t nb, uint32_t key) {
+    if (nb <= 0) return false;
+    int32_t base = 0, n = nb;
+    while (n > 1) {
+        int32_t half = n >> 1;
+        base = (b[base + half] <= key) ? base + half : base;
+        n -= half;
+    }
+    return b[base] == key;
+}
+
+// Clique rule at depth n-2: the count is sum over v in L of |L ∩ N(v)|, the
+// size of the last loop set each candidate would build
+// (gpu/gpu_clique.cu:224-250).  L is staged in LDS and every lane takes its
+// own candidates, iterating the shorter of L and N(v) and searching the
+// other.  Per candidate the byte model charges what the wave path does:
+// 12 B (element + row pointers) + 4 (|L| + |N(v)|) for a size-only
+// intersection.
+__device__ void clique_tail_count(Wave& w, SetRef L) {
+    const int ln = lane_id();
+    for (int32_t i = ln; i < L.len; i += kWave) w.stage[i] = L.ptr[i];
+    wave_sync();
+    unsigned long long acc = 0, bytes = 0;
+    for (int32_t i = ln; i < L.len; i += kWave) {
+        uint32_t v = w.stage[i];
+        int64_t l = w.a.row[v], r = w.a.row[v + 1];
+        int32_t d = (int32_t)(r - l);
+        const uint32_t* nb = w.a.col + l;
+        int32_t c = 0;
+        if (d <= L.len) {
+            for (int32_t j = 0; j < d; ++j) c += lds_contains(w.stage, L.len, nb[j]) ? 1 : 0;
+        } else {
+            for (int32_t j = 0; j < L.len; ++j) c += contains(nb, d, w.stage[j]) ? 1 : 0;
+        }
+        acc += (unsigned long long)c;
+        bytes += 12ull + 4ull * (unsigned long long)(L.len + d);
+    }
+    w.lane_sum += acc;
+    if (w.a.stats) w.lane_bytes += bytes;
+    wave_sync();
+}
+
 template <int D, int MODE>
 __device__ void dfs(Wave& w) {
     if constexpr (D >= GM_MAX_PATTERN) {
@@ -345,6 +403,10 @@ __device__ void dfs(Wave& w) {
             int32_t lim = unii(lower_bound_u(loop.ptr, loop.len, bound));
             w.sum += (unsigned long long)(long long)subtract_emb<D>(w, loop.ptr, lim);
             if (w.a.stats) w.bytes += 4ull * D;
+            return;
+        }
+        if (MODE == GM_MODE_CLIQUE && D == n - 2 && p.clique_tail && loop.len <= kStage) {
+            clique_tail_count(w, loop);
             return;
         }
         const uint32_t bound = (MODE == GM_MODE_PATTERN) ? restriction_bound<D>(w) : 0xffffffffu;
@@ -408,18 +470,33 @@ __device__ void process_root(Wave& w, uint32_t v0, uint32_t v1) {
     dfs<2, MODE>(w);
 }
 
+// Sum of a per-lane value over the wave (all lanes receive it).
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+        lo = __shfl_xor(lo, off);
+        hi = __shfl_xor(hi, off);
+        x += ((unsigned long long)hi << 32) | lo;
+    }
+    return x;
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
-    __shared__ WaveLds lds_all[kWavesPerBlock];
+    __shared__ WaveShared<MODE> lds_all[kWavesPerBlock];
     Wave w;
     w.a = a;
     w.pl = a.plan;
     const int wid = threadIdx.x / kWave;
-    w.lds = &lds_all[wid];
+    w.lds = &lds_all[wid].w;
+    w.stage = lds_all[wid].stage;
     const int64_t gwave = (int64_t)blockIdx.x * kWavesPerBlock + wid;
     w.scratch = a.scratch + gwave * a.wave_stride;
     w.sum = 0;
     w.bytes = 0;
+    w.lane_sum = 0;
+    w.lane_bytes = 0;
     w.overflow = 0;
 #pragma unroll
     for (int i = 0; i <= GM_MAX_PATTERN; ++i) w.emb[i] = 0;
@@ -445,6 +522,8 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
             v1 = a.col[e];
             v0 = (uint32_t)edge_source(a.row, a.nv, e);
             if (restricted1 && v0 <= v1) ok = false;
+            if (MODE == GM_MODE_CLIQUE && a.root_min_outdeg > 0 && a.row[v0 + 1] - a.row[v0] <= a.root_min_outdeg)
+                ok = false;
         }
         uint64_t m = ballot(ok);
         while (m != 0) {
@@ -456,11 +535,246 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
             process_root<MODE>(w, u0, u1);
         }
     }
+    const unsigned long long lane_total = wave_sum(w.lane_sum);
+    const unsigned long long lane_bytes = a.stats ? wave_sum(w.lane_bytes) : 0ull;
     if (ln == 0) {
-        atomicAdd(&a.ctr->sum, w.sum);
+        atomicAdd(&a.ctr->sum, w.sum + lane_total);
         atomicAdd(&a.ctr->tasks, tasks);
-        if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes);
+        if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes + lane_bytes);
         if (w.overflow) atomicOr(&a.ctr->overflow, 1u);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Local-bitmap clique engine (GM_MODE_CLIQUE, plain k-clique chain).
+//
+// For a root v0 of out-degree d <= kBmMaxD every later candidate set is a
+// subset of N(v0), so the enumeration below v0 runs on the d-vertex local
+// DAG: row i is the bitmap (over positions in N(v0)) of N(N[i]) ∩ N(v0),
+// built once per root.  Then a k-clique through v0 is a chain i -> j -> ...
+// and every set operation of the schedule is a W-word AND (W = ceil(d/64)):
+//   |S_{v0,i,j}|            = popcount(row_i & row_j)
+//   |S_{v0,i,j} ∩ N(l)|     = popcount(row_i & row_j & row_l), l in row_i & row_j
+// The (i, j) edges of the local DAG are spread evenly over the lanes through
+// an exclusive scan of the row popcounts.  This is the local-subgraph bitmap
+// idea of the reference's gpu/gpu_kclique.cu:359-400, applied to the
+// schedule's clique chain; counts equal the list engine's.
+//
+// Byte model (HBM, per root): 16 (row pointers) + 4 d (N(v0)) +
+// sum over u in N(v0) of (8 + 4 |N(u)|) for building the rows; the counting
+// phase reads only the rows (on chip).
+// ---------------------------------------------------------------------------
+constexpr int kBmMaxD = 1024;      // largest root out-degree handled with bitmaps
+constexpr int kBmMaxW = kBmMaxD / 64;
+constexpr int kBmLdsWords = 1024;  // rows kept in LDS when d * W <= this
+
+struct BmArgs {
+    const int64_t* row;
+    const uint32_t* col;
+    int32_t nv;
+    int32_t k;                 // clique size
+    int32_t max_d;             // roots with out-degree above this are left to the list engine
+    unsigned long long* rows_scratch;  // per wave kBmMaxD * kBmMaxW words
+    int64_t nchunks;           // root chunks of kBmChunk vertices
+    Counters* ctr;
+    int32_t stats;
+};
+constexpr int kBmChunk = 16;
+
+struct BmShared {
+    uint32_t nbr[kBmMaxD];
+    int32_t off[kBmMaxD + 1];
+    unsigned long long rows[kBmLdsWords];
+};
+
+// Position of the r-th set bit (0-based) of x; x has more than r set bits.
+__device__ __forceinline__ int select_bit64(unsigned long long x, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int width = 32; width >= 1; width >>= 1) {
+        unsigned long long low = x & ((width == 64 ? ~0ull : ((1ull << width) - 1)));
+        int c = __popcll(low);
+        if (r >= c) {
+            r -= c;
+            x >>= width;
+            pos += width;
+        }
+    }
+    return pos;
+}
+
+template <int WMAX, int K>
+__device__ unsigned long long bm_count_pair(const unsigned long long* rows, int W, int i, int j) {
+    unsigned long long S[WMAX];
+#pragma unroll
+    for (int w = 0; w < WMAX; ++w) S[w] = (w < W) ? (rows[i * W + w] & rows[j * W + w]) : 0ull;
+    unsigned long long acc = 0;
+    if constexpr (K == 4) {
+#pragma unroll
+        for (int w = 0; w < WMAX; ++w) acc += __popcll(S[w]);
+    } else {
+#pragma unroll
+        for (int w = 0; w < WMAX; ++w) {
+            unsigned long long m = S[w];
+            while (m) {
+                int l = w * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+                const unsigned long long* rl = rows + (int64_t)l * W;
+                if constexpr (K == 5) {
+#pragma unroll
+                    for (int x = 0; x < WMAX; ++x)
+                        if (x < W) acc += __popcll(S[x] & rl[x]);
+                } else {
+                    unsigned long long T[WMAX];
+#pragma unroll
+                    for (int x = 0; x < WMAX; ++x) T[x] = (x < W) ? (S[x] & rl[x]) : 0ull;
+#pragma unroll
+                    for (int x = 0; x < WMAX; ++x) {
+                        unsigned long long mm = T[x];
+                        while (mm) {
+                            int q = x * 64 + __builtin_ctzll(mm);
+                            mm &= mm - 1;
+                            const unsigned long long* rq = rows + (int64_t)q * W;
+                            if constexpr (K == 6) {
+#pragma unroll
+                                for (int y = 0; y < WMAX; ++y)
+                                    if (y < W) acc += __popcll(T[y] & rq[y]);
+                            } else {  // K == 7
+#pragma unroll
+                                for (int y = 0; y < WMAX; ++y) {
+                                    if (y >= W) continue;
+                                    unsigned long long mq = T[y] & rq[y];
+                                    while (mq) {
+                                        int z = y * 64 + __builtin_ctzll(mq);
+                                        mq &= mq - 1;
+                                        const unsigned long long* rz = rows + (int64_t)z * W;
+#pragma unroll
+                                        for (int t = 0; t < WMAX; ++t)
+                                            if (t < W) acc += __popcll(T[t] & rq[t] & rz[t]);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return acc;
+}
+
+template <int WMAX, int K>
+__device__ unsigned long long bm_root(const BmArgs& a, BmShared& sh, unsigned long long* gl_rows, uint32_t v0,
+                                      int64_t l0, int32_t d, unsigned long long& bytes) {
+    const int ln = lane_id();
+    const int W = (d + 63) >> 6;
+    for (int i = ln; i < d; i += kWave) sh.nbr[i] = a.col[l0 + i];
+    unsigned long long* rows = (d * W <= kBmLdsWords) ? sh.rows : gl_rows;
+    for (int t = ln; t < d * W; t += kWave) rows[t] = 0ull;
+    wave_sync();
+    // Row i = positions in N(v0) of N(N[i]); lane-private rows, no conflicts.
+    unsigned long long my_bytes = 0;
+    for (int i = ln; i < d; i += kWave) {
+        uint32_t u = sh.nbr[i];
+        int64_t lu = a.row[u], ru = a.row[u + 1];
+        int32_t du = (int32_t)(ru - lu);
+        my_bytes += 8ull + 4ull * (unsigned long long)du;
+        for (int32_t t = 0; t < du; ++t) {
+            uint32_t x = a.col[lu + t];
+            int32_t base = 0, n = d;
+            while (n > 1) {
+                int32_t half = n >> 1;
+                base = (sh.nbr[base + half] <= x) ? base + half : base;
+                n -= half;
+            }
+            if (sh.nbr[base] == x) rows[i * W + (base >> 6)] |= 1ull << (base & 63);
+        }
+    }
+    bytes += my_bytes;
+    wave_sync();
+    unsigned long long acc = 0;
+    if constexpr (K == 3) {
+        for (int t = ln; t < d * W; t += kWave) acc += __popcll(rows[t]);
+        return acc;
+    } else {
+        // Exclusive scan of row popcounts -> off[0..d].
+        int32_t carry = 0;
+        for (int base = 0; base < d; base += kWave) {
+            int i = base + ln;
+            int32_t c = 0;
+            if (i < d)
+                for (int w = 0; w < W; ++w) c += __popcll(rows[i * W + w]);
+            int32_t incl = c;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                int32_t y = __shfl_up(incl, off);
+                if (ln >= off) incl += y;
+            }
+            if (i < d) sh.off[i] = carry + incl - c;
+            carry += __shfl(incl, kWave - 1);
+        }
+        if (ln == 0) sh.off[d] = carry;
+        wave_sync();
+        const int32_t P = carry;
+        for (int32_t pidx = ln; pidx < P; pidx += kWave) {
+            // Row i with off[i] <= pidx < off[i+1].
+            int32_t lo = 0, n = d;
+            while (n > 1) {
+                int32_t half = n >> 1;
+                lo = (sh.off[lo + half] <= pidx) ? lo + half : lo;
+                n -= half;
+            }
+            const int i = lo;
+            int r = pidx - sh.off[i];
+            int j = 0;
+            for (int w = 0; w < W; ++w) {
+                unsigned long long word = rows[i * W + w];
+                int c = __popcll(word);
+                if (r < c) {
+                    j = w * 64 + select_bit64(word, r);
+                    break;
+                }
+                r -= c;
+            }
+            acc += bm_count_pair<WMAX, K>(rows, W, i, j);
+        }
+        wave_sync();
+        return acc;
+    }
+}
+
+template <int WMAX, int K>
+__global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
+    __shared__ BmShared sh;
+    const int ln = lane_id();
+    unsigned long long* gl_rows = a.rows_scratch + (int64_t)blockIdx.x * kBmMaxD * kBmMaxW;
+    unsigned long long acc = 0, bytes = 0, wave_bytes = 0;
+    unsigned long long roots = 0;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (ln == 0) chunk = atomicAdd(&a.ctr->cursor, 1ull);
+        chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
+        if ((long long)chunk >= a.nchunks) break;
+        const int32_t vbeg = (int32_t)(chunk * kBmChunk);
+        const int32_t vend = vbeg + kBmChunk < a.nv ? vbeg + kBmChunk : a.nv;
+        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
+            int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
+            int32_t d = unii((int32_t)(r0 - l0));
+            if (d < K - 1 || d > a.max_d) continue;
+            ++roots;
+            if (a.stats) wave_bytes += 16ull + 4ull * (unsigned long long)d;
+            if (d <= 64) acc += bm_root<1, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
+            else if (d <= 256) acc += bm_root<4, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
+            else acc += bm_root<WMAX, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
+        }
+    }
+    const unsigned long long total = wave_sum(acc);
+    const unsigned long long tb = a.stats ? wave_sum(bytes) : 0ull;
+    if (ln == 0) {
+        atomicAdd(&a.ctr->sum, total);
+        atomicAdd(&a.ctr->tasks, roots);
+        if (a.stats) atomicAdd(&a.ctr->bytes, tb + wave_bytes);
     }
 }
 
@@ -489,6 +803,8 @@ struct DeviceCache {
     std::mutex mu;
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    void* bm_rows = nullptr;
+    size_t bm_bytes = 0;
     Counters* ctr = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -500,6 +816,22 @@ DeviceCache& cache_for(int dev) {
     if ((int)caches.size() <= dev) caches.resize(dev + 1, nullptr);
     if (!caches[dev]) caches[dev] = new DeviceCache();
     return *caches[dev];
+}
+
+// One 64-lane block per resident slot of the bitmap kernel (its LDS bounds
+// residency at 10 blocks per CU).
+int bm_blocks(int cus) { return cus * 10; }
+
+hipError_t launch_bitmap(int k, int blocks, const BmArgs& b, hipStream_t s) {
+    switch (k) {
+        case 3: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 3>), dim3(blocks), dim3(kWave), 0, s, b); break;
+        case 4: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 4>), dim3(blocks), dim3(kWave), 0, s, b); break;
+        case 5: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 5>), dim3(blocks), dim3(kWave), 0, s, b); break;
+        case 6: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 6>), dim3(blocks), dim3(kWave), 0, s, b); break;
+        case 7: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 7>), dim3(blocks), dim3(kWave), 0, s, b); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -690,6 +1022,32 @@ int gm_plan_create(const gm_plan_desc* d, gm_plan** out) {
             if (h.father[q] >= 0) h.size_only[h.father[q]] = 0;
         for (int dd = 1; dd < n - k; ++dd) h.size_only[h.loopset[dd]] = 0;
     }
+    if (h.mode == GM_MODE_CLIQUE && n >= 3) {
+        // Only the size of the last loop set is read (gpu/gpu_clique.cu:229-232).
+        int q = h.loopset[n - 1];
+        bool has_child = false;
+        for (int c = 0; c < P; ++c) has_child |= (h.father[c] == q);
+        bool earlier = false;
+        for (int dd = 1; dd < n - 1; ++dd) earlier |= (h.loopset[dd] == q);
+        if (h.father[q] >= 0 && !has_child && !earlier) h.size_only[q] = 1;
+        // Depth n-2 can be counted lane-parallel when it builds exactly that
+        // set, from its own loop set.
+        if (n >= 4 && h.size_only[q] && h.last[n - 2] == q && h.next[q] == -1 && h.father[q] == h.loopset[n - 2])
+            h.clique_tail = 1;
+        // The plain chain: complete pattern, one prefix per depth, each the
+        // father of the next, prefix of depth d the loop set of d+1.
+        bool chain = h.nrestrict == 0;
+        for (int i = 0; i < n && chain; ++i)
+            for (int j = 0; j < n; ++j)
+                if (i != j && !h.adj[i * n + j]) chain = false;
+        int prev = -1;
+        for (int dd = 0; dd + 1 < n && chain; ++dd) {
+            int qd = h.last[dd];
+            if (qd < 0 || h.next[qd] != -1 || h.father[qd] != prev || h.loopset[dd + 1] != qd) chain = false;
+            prev = qd;
+        }
+        h.clique_chain = chain ? 1 : 0;
+    }
     gm_plan* p = new gm_plan();
     p->host = h;
     p->d_plan = nullptr;
@@ -737,8 +1095,16 @@ int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, c
     std::lock_guard<std::mutex> lock(dc.mu);
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-    int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : 8;
-    int blocks = std::max(1, cus * waves_per_cu / kWavesPerBlock);
+    // Default grid: every block the occupancy of either kernel allows
+    // (persistent waves; the kernels exit when the root cursor runs out).
+    int occ_pattern = 0, occ_clique = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pattern, enumerate_kernel<GM_MODE_PATTERN>, kBlock, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_clique, enumerate_kernel<GM_MODE_CLIQUE>, kBlock, 0));
+    bool any_clique = false;
+    for (int i = 0; i < np; ++i) any_clique |= plans[i]->host.mode == GM_MODE_CLIQUE;
+    int max_blocks_per_cu = std::max(1, any_clique ? std::min(occ_pattern, occ_clique) : occ_pattern);
+    int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : max_blocks_per_cu * kWavesPerBlock;
+    int blocks = std::max(1, cus * std::min(waves_per_cu, max_blocks_per_cu * kWavesPerBlock) / kWavesPerBlock);
     const int64_t nwaves = (int64_t)blocks * kWavesPerBlock;
     const int32_t cap = std::max(1, g->second_deg);
     const int32_t cap_vi = std::max(1, g->max_deg);
@@ -775,7 +1141,38 @@ int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, c
         a.nchunks = (g->nnz + kWave - 1) / kWave;
         a.ctr = dc.ctr;
         a.stats = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
+        a.root_min_outdeg = 0;
+        const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
+        const bool use_bitmap = p->host.mode == GM_MODE_CLIQUE && p->host.clique_chain && bm_max > 0;
+        Counters c_bm;
+        std::memset(&c_bm, 0, sizeof(c_bm));
         HIP_TRY(hipEventRecord(dc.ev0, s));
+        if (use_bitmap) {
+            // Roots of out-degree <= kBmMaxD: local-bitmap engine; the rest:
+            // the list engine below.
+            const size_t bm_need = (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long);
+            if (bm_need > dc.bm_bytes) {
+                if (dc.bm_rows) HIP_TRY(hipFree(dc.bm_rows));
+                dc.bm_rows = nullptr;
+                dc.bm_bytes = 0;
+                HIP_TRY(hipMalloc(&dc.bm_rows, bm_need));
+                dc.bm_bytes = bm_need;
+            }
+            BmArgs b;
+            b.row = g->d_row;
+            b.col = g->d_col;
+            b.nv = g->n;
+            b.k = n;
+            b.max_d = bm_max;
+            b.rows_scratch = (unsigned long long*)dc.bm_rows;
+            b.nchunks = ((int64_t)g->n + kBmChunk - 1) / kBmChunk;
+            b.ctr = dc.ctr;
+            b.stats = a.stats;
+            HIP_TRY(launch_bitmap(n, bm_blocks(cus), b, s));
+            HIP_TRY(hipMemcpyAsync(&c_bm, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemsetAsync(dc.ctr, 0, sizeof(Counters), s));
+            a.root_min_outdeg = bm_max;
+        }
         if (p->host.mode == GM_MODE_CLIQUE)
             hipLaunchKernelGGL(enumerate_kernel<GM_MODE_CLIQUE>, dim3(blocks), dim3(kBlock), 0, s, a);
         else
@@ -788,10 +1185,10 @@ int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, c
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
         if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
-        raw_sums[pi] = c.sum;
+        raw_sums[pi] = c.sum + c_bm.sum;
         total_ms += ms;
-        total_bytes += c.bytes;
-        total_tasks += c.tasks;
+        total_bytes += c.bytes + c_bm.bytes;
+        total_tasks += c.tasks + c_bm.tasks;
     }
     if (stats) {
         stats->kernel_ms = total_ms;

@@ -104,6 +104,13 @@ GS_API int64_t gs_graph_count_triangles(void* gp) { return count_triangles(*(Gra
 
 GS_API void gs_graph_reduce_for_clique(void* gp) { reduce_edges_for_clique(*(Graph*)gp); }
 
+GS_API void* gs_graph_orient_degree(void* gp) {
+    Graph* out = nullptr;
+    degree_orientation_init((Graph*)gp, out);
+    fflush(stdout);
+    return out;
+}
+
 GS_API void gs_set_max_intersection_size(int32_t m) { VertexSet::max_intersection_size = m; }
 
 // ---- schedules -------------------------------------------------------------

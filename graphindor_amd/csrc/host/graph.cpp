@@ -2,7 +2,9 @@
 // gpu/gpu_mc.cu:29-59, gpu/gpu_clique.cu:366-570).
 #include "graph.h"
 
+#include <algorithm>
 #include <chrono>
+#include <vector>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -176,6 +178,35 @@ void reduce_edges_for_clique(Graph& g) {
     printf("newe: %ld\n", (long)g.e_cnt);
     printf("Finish reduce.\n");
     g.invalidate_device();
+}
+
+void degree_orientation_init(Graph* original_g, Graph*& g) {
+    const v_index_t n = original_g->v_cnt;
+    g = new Graph();
+    g->v_cnt = n;
+    g->tri_cnt = original_g->tri_cnt;
+    g->vertex = new e_index_t[(size_t)n + 1];
+    std::vector<e_index_t> deg(n);
+    for (v_index_t u = 0; u < n; ++u) deg[u] = original_g->vertex[u + 1] - original_g->vertex[u];
+    auto forward = [&](v_index_t u, v_index_t v) { return deg[u] < deg[v] || (deg[u] == deg[v] && u < v); };
+    e_index_t total = 0;
+    for (v_index_t u = 0; u < n; ++u) {
+        g->vertex[u] = total;
+        for (e_index_t i = original_g->vertex[u]; i < original_g->vertex[u + 1]; ++i)
+            if (forward(u, original_g->edge[i])) ++total;
+    }
+    g->vertex[n] = total;
+    g->e_cnt = total;
+    g->edge = new v_index_t[total > 0 ? (size_t)total : 1];
+    e_index_t max_out = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : max_out)
+    for (v_index_t u = 0; u < n; ++u) {
+        e_index_t p = g->vertex[u];
+        for (e_index_t i = original_g->vertex[u]; i < original_g->vertex[u + 1]; ++i)
+            if (forward(u, original_g->edge[i])) g->edge[p++] = original_g->edge[i];
+        max_out = std::max(max_out, p - g->vertex[u]);
+    }
+    printf("Maximum degree after orientation: %ld\n", (long)max_out);
 }
 
 unsigned long long gpu_pattern_matching(Graph* g, const Schedule_IEP& s, GpuRunInfo* info) {

@@ -60,6 +60,12 @@ private:
 // (src/graph.cpp:710-735).
 void reduce_edges_for_clique(Graph& g);
 
+// Orient every undirected edge of a symmetric graph from the endpoint with
+// the smaller (degree, id) to the other; `g` receives a new graph with half
+// the CSR entries (src/graph.cpp:821-859).  Each k-clique is then one chain
+// of out-neighbours, and out-degrees stay near sqrt(2|E|) on skewed graphs.
+void degree_orientation_init(Graph* original_g, Graph*& g);
+
 // Device drivers replacing the file-local ones of gpu/gpu_graph.cu:22-51,
 // gpu/gpu_mc.cu:29-59 and gpu/gpu_clique.cu:366-570.  They return the count
 // after the division by the IEP redundancy; `info` (optional) receives the

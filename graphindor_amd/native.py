@@ -52,7 +52,8 @@ class GmPlanDesc(ctypes.Structure):
 
 class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
-                ("lds_set_cap", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class GmStats(ctypes.Structure):
@@ -117,6 +118,7 @@ def gs():
             ("gs_graph_set_tri_cnt", [VP, ctypes.c_int64], None),
             ("gs_graph_count_triangles", [VP], ctypes.c_int64),
             ("gs_graph_reduce_for_clique", [VP], None),
+            ("gs_graph_orient_degree", [VP], VP),
             ("gs_set_max_intersection_size", [ctypes.c_int32], None),
             ("gs_set_restrict_validation", [ctypes.c_int], None),
             ("gs_schedule_new", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -213,6 +215,10 @@ class Graph:
 
     def reduce_for_clique(self):
         gs().gs_graph_reduce_for_clique(self.h)
+
+    def orient_by_degree(self):
+        """New graph: each edge once, from the smaller (degree, id) end."""
+        return Graph(gs().gs_graph_orient_degree(self.h))
 
     def set_tri_cnt(self, t):
         gs().gs_graph_set_tri_cnt(self.h, t)
@@ -335,12 +341,13 @@ class DevicePlan:
             pass
 
 
-def count(dgraph, dplan, flags=0, waves_per_cu=0):
+def count(dgraph, dplan, flags=0, waves_per_cu=0, bitmap_max_degree=0):
     """Raw sum (before the redundancy division) and the gm_stats, via gm_count."""
     raw = ctypes.c_uint64(0)
     st = GmStats()
     tun = GmTuning()
     tun.waves_per_cu = waves_per_cu
+    tun.bitmap_max_degree = bitmap_max_degree
     _gm_check(gm().gm_count(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw), ctypes.byref(st)),
               "gm_count")
     return raw.value, st

@@ -89,10 +89,12 @@ typedef struct gm_plan_desc {
 
 /* Launch tuning; zero fields mean "library default". */
 typedef struct gm_tuning {
-    int32_t waves_per_cu;   /* resident waves per CU of the persistent grid */
-    int32_t chunk_edges;    /* root edges claimed per atomic work-steal */
-    int32_t lds_set_cap;    /* max elements of a prefix set staged in LDS */
-    int32_t reserved[5];
+    int32_t waves_per_cu;      /* resident waves per CU of the persistent grid */
+    int32_t chunk_edges;       /* root edges claimed per atomic work-steal */
+    int32_t lds_set_cap;       /* max elements of a prefix set staged in LDS */
+    int32_t bitmap_max_degree; /* GM_MODE_CLIQUE: roots of out-degree up to this use the
+                                  local-bitmap engine (0 = default 1024, -1 = off) */
+    int32_t reserved[4];
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */

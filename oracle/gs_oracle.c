@@ -69,6 +69,7 @@ typedef struct {
     uint64_t bytes;
     uint8_t size_only[GM_MAX_PREFIX];
     int gpu_rule;     /* 1: edge-rooted device rule, 0: CPU engine rule */
+    int32_t d1_stride, d1_offset; /* CPU rule: expand only depth-1 candidates i = offset mod stride */
 } or_ctx;
 
 static int64_t max_degree(const or_graph* g) {
@@ -85,6 +86,14 @@ static void derive_size_only(const gm_plan_desc* p, uint8_t* out) {
     int P = p->total_prefix_num;
     int tail = p->size - p->in_exclusion_optimize_num;
     for (int i = 0; i < P; ++i) out[i] = 0;
+    if (p->mode == GM_MODE_CLIQUE) {
+        /* only the size of the last loop set is read */
+        int q = p->loop_set_prefix_id[p->size - 1], used = 0;
+        for (int c = 0; c < P; ++c) used |= (p->father_prefix_id[c] == q);
+        for (int d = 1; d < p->size - 1; ++d) used |= (p->loop_set_prefix_id[d] == q);
+        if (q >= 0 && p->father_prefix_id[q] >= 0 && !used) out[q] = 1;
+        return;
+    }
     if (p->in_exclusion_optimize_num == 0) return;
     for (int i = 0; i < p->iep_vertex_num; ++i) out[p->iep_vertex_id[i]] = 1;
     for (int i = 0; i < p->iep_vertex_num; ++i)
@@ -241,9 +250,11 @@ static void dfs(or_ctx* c, int depth) {
         return;
     }
     uint32_t bound = restriction_bound(c, depth, c->gpu_rule ? 0xffffffffu : (uint32_t)c->g->n);
+    const int sampled = depth == 1 && c->d1_stride > 1;
     for (int32_t i = 0; i < loop.len; ++i) {
         uint32_t v = (uint32_t)loop.ptr[i];
         if (bound <= v) break;
+        if (sampled && i % c->d1_stride != c->d1_offset) continue;
         c->bytes += 4ull;
         if (in_emb(c, v)) continue;
         int64_t l = c->g->vertex[v], r = c->g->vertex[v + 1];
@@ -283,20 +294,22 @@ static int plan_ok(const gm_plan_desc* p) {
            p->iep_vertex_num <= GM_MAX_IEP && p->iep_array_size <= GM_MAX_IEP;
 }
 
-/* Raw sum (before the redundancy division) with the CPU engine's rule over
- * the root vertices v0 = offset, offset + stride, ...  (stride 1: all). */
+/* Raw sum (before the redundancy division) with the CPU engine's rule,
+ * expanding only the depth-1 candidates with index = offset mod stride of
+ * every root (stride 1: everything).  Summed over all offsets this is the
+ * full count; one offset is an even sample of the work, hub roots included. */
 uint64_t oracle_count_vertex_rooted_strided(const or_graph* g, const gm_plan_desc* p, int threads, int32_t stride,
                                             int32_t offset) {
     if (!plan_ok(p) || stride < 1) return 0;
     uint64_t total = 0;
-    int32_t nroots = offset < g->n ? (g->n - offset + stride - 1) / stride : 0;
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel reduction(+ : total)
     {
         or_ctx* c = ctx_new(g, p, 0);
+        c->d1_stride = stride;
+        c->d1_offset = offset % stride;
 #pragma omp for schedule(dynamic)
-        for (int32_t r = 0; r < nroots; ++r) {
-            int32_t v0 = offset + r * stride;
+        for (int32_t v0 = 0; v0 < g->n; ++v0) {
             const int32_t* nb = g->edge + g->vertex[v0];
             int32_t d = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
             for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, nb, d);
@@ -496,4 +509,60 @@ uint64_t oracle_brute_force(const or_graph* g, int n, const int32_t* adj, int in
         total += c.count;
     }
     return total / automorphisms(n, adj);
+}
+
+/* The clique rule split the way the device splits it: roots v0 of out-degree
+ * d <= dmax go to the local-bitmap engine, whose byte model is
+ * 16 + 4 d + sum over u in N(v0) of (8 + 4 |N(u)|) for roots with d >= k-1;
+ * the edges of the other roots take the list rule with its own model.  The
+ * count is the clique rule's either way. */
+uint64_t oracle_count_clique_hybrid(const or_graph* g, const gm_plan_desc* p, int threads, int32_t dmax,
+                                    uint64_t* alg_bytes) {
+    gm_plan_desc q = *p;
+    q.mode = GM_MODE_CLIQUE;
+    int k = p->size;
+    uint64_t total = 0, bytes = 0;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel reduction(+ : total, bytes)
+    {
+        or_ctx* c = ctx_new(g, &q, 1);
+#pragma omp for schedule(dynamic, 64)
+        for (int32_t v0 = 0; v0 < g->n; ++v0) {
+            int64_t l0 = g->vertex[v0], r0 = g->vertex[v0 + 1];
+            int32_t d0 = (int32_t)(r0 - l0);
+            int small = d0 <= dmax;
+            if (small && d0 < k - 1) continue;
+            uint64_t b0 = c->bytes;
+            for (int64_t e = l0; e < r0; ++e) {
+                uint32_t v1 = (uint32_t)g->edge[e];
+                c->bytes += 24ull;
+                const int32_t* n0 = g->edge + l0;
+                for (int pid = q.last[0]; pid != -1; pid = q.next[pid]) build(c, pid, n0, d0);
+                const int32_t* n1 = g->edge + g->vertex[v1];
+                int32_t d1 = (int32_t)(g->vertex[v1 + 1] - g->vertex[v1]);
+                int zero = 0;
+                for (int pid = q.last[1]; pid != -1; pid = q.next[pid]) {
+                    build(c, pid, n1, d1);
+                    if (c->sets[pid].len == 0 && pid < q.basic_prefix_num) { zero = 1; break; }
+                }
+                if (zero) continue;
+                c->emb[0] = (uint32_t)v0;
+                c->emb[1] = v1;
+                c->emb_n = 2;
+                oracle_clique_dfs(c, 2);
+            }
+            if (small) {
+                c->bytes = b0 + 16ull + 4ull * (uint64_t)d0;
+                for (int64_t e = l0; e < r0; ++e) {
+                    uint32_t u = (uint32_t)g->edge[e];
+                    c->bytes += 8ull + 4ull * (uint64_t)(g->vertex[u + 1] - g->vertex[u]);
+                }
+            }
+        }
+        total += c->sum;
+        bytes += c->bytes;
+        ctx_free(c);
+    }
+    if (alg_bytes) *alg_bytes = bytes;
+    return total;
 }

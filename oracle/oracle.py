@@ -38,6 +38,9 @@ def lib():
             getattr(L, name).argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]
             getattr(L, name).restype = ctypes.c_uint64
+        L.oracle_count_clique_hybrid.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
+                                                  ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_count_clique_hybrid.restype = ctypes.c_uint64
         L.oracle_brute_force.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                          ctypes.c_int]
         L.oracle_brute_force.restype = ctypes.c_uint64
@@ -105,6 +108,13 @@ def count_clique(csr, desc_ptr, threads=0):
     b = ctypes.c_uint64(0)
     s = lib().oracle_count_clique(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p), threads,
                                   ctypes.byref(b))
+    return s, b.value
+
+
+def count_clique_hybrid(csr, desc_ptr, dmax, threads=0):
+    b = ctypes.c_uint64(0)
+    s = lib().oracle_count_clique_hybrid(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p), threads, dmax,
+                                         ctypes.byref(b))
     return s, b.value
 
 

@@ -122,3 +122,18 @@ def test_reduce_for_clique_keeps_lower_neighbours():
     v2, e2 = g.csr()
     rv, re_ = reduce_for_clique(v, e)
     assert np.array_equal(v2, rv) and np.array_equal(e2, re_)
+
+
+def test_degree_orientation():
+    from tests.common import random_graph
+    v, e = random_graph(50, 0.2, 4)
+    g = N.Graph.from_csr(v, e)
+    d = g.orient_by_degree()
+    dv, de = d.csr()
+    assert len(de) * 2 == len(e)
+    deg = np.diff(v)
+    src = np.repeat(np.arange(len(v) - 1), np.diff(dv))
+    assert np.all((deg[src] < deg[de]) | ((deg[src] == deg[de]) & (src < de)))
+    pairs = set(zip(src.tolist(), de.tolist()))
+    full = set(zip(np.repeat(np.arange(len(v) - 1), deg).tolist(), e.tolist()))
+    assert {tuple(sorted(p)) for p in pairs} == {tuple(sorted(p)) for p in full}

@@ -23,19 +23,30 @@ def _gpu():
     require_gpu()
 
 
+BITMAP_DMAX = 1024  # the device's default local-bitmap bound (kBmMaxD)
+
+
 def gpu_vs_oracle(v, e, sched, mode=N.GM_MODE_PATTERN, waves_per_cu=0):
+    """GPU == oracle on raw sum and byte tally; for clique plans both with the
+    local-bitmap engine (default) and without it (list engine only)."""
     print(f"case: n={len(v) - 1} nnz={len(e)} mode={mode}", flush=True)
     desc = sched.plan_desc(mode)
     dg = N.DeviceGraph(v, e)
     dp = N.DevicePlan(desc)
-    raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, waves_per_cu=waves_per_cu)
     csr = O.CSR(v, e)
     if mode == N.GM_MODE_CLIQUE:
-        oraw, ob = O.count_clique(csr, desc)
+        raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, waves_per_cu=waves_per_cu)
+        oraw, ob = O.count_clique_hybrid(csr, desc, BITMAP_DMAX)
+        assert (raw, st.alg_bytes) == (oraw, ob)
+        raw_l, st_l = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, waves_per_cu=waves_per_cu, bitmap_max_degree=-1)
+        oraw_l, ob_l = O.count_clique(csr, desc)
+        assert (raw_l, st_l.alg_bytes) == (oraw_l, ob_l)
+        assert raw_l == raw
     else:
+        raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, waves_per_cu=waves_per_cu)
         oraw, ob = O.count_edge_rooted(csr, desc)
-    assert raw == oraw
-    assert st.alg_bytes == ob
+        assert raw == oraw
+        assert st.alg_bytes == ob
     raw2, _ = N.count(dg, dp)          # without the byte tally: same sum
     assert raw2 == raw
     return raw // sched.redundancy
@@ -181,3 +192,82 @@ def test_reference_mode_plans_match_oracle():
             gpu_vs_oracle(v, e, s)
     finally:
         N.set_restrict_validation("exact")
+
+
+def test_degree_oriented_cliques():
+    """The clique rule on degree-oriented graphs (reference
+    degree_orientation_init) counts what the id-oriented reduction counts."""
+    for scale, seed in ((12, 4), (14, 1)):
+        g = N.Graph.rmat(scale, 16, seed)
+        d = g.orient_by_degree()
+        dv, de = d.csr()
+        dv, de = dv.copy(), de.copy()
+        g.reduce_for_clique()
+        rv, re_ = g.csr()
+        rv, re_ = rv.copy(), re_.copy()
+        for k in (3, 4, 5, 6):
+            s = N.Schedule.plain(k, clique_adj(k))
+            s.redundancy = 1
+            a = gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)
+            b = gpu_vs_oracle(rv, re_, s, N.GM_MODE_CLIQUE)
+            assert a == b
+
+
+def test_clique_tail_with_large_loop_sets():
+    """Two apex vertices over 2100 sparse leaves: the root edge between the
+    apexes has a loop set larger than the LDS staging arena, which takes the
+    wave-wide fallback; counts and bytes still match the oracle."""
+    import random as _r
+    rng = _r.Random(9)
+    leaves = 2100
+    adj = [set() for _ in range(leaves + 2)]
+    for a in range(leaves):
+        for b in range(a + 1, min(leaves, a + 40)):
+            if rng.random() < 0.1:
+                adj[a].add(b)
+                adj[b].add(a)
+    for apex in (leaves, leaves + 1):
+        for a in range(leaves):
+            adj[apex].add(a)
+            adj[a].add(apex)
+    adj[leaves].add(leaves + 1)
+    adj[leaves + 1].add(leaves)
+    from tests.common import csr_from_adjsets
+    v, e = csr_from_adjsets(adj)
+    rv, re_ = reduce_for_clique(v, e)
+    for k in (4, 5):
+        s = N.Schedule.plain(k, clique_adj(k))
+        s.redundancy = 1
+        gpu_vs_oracle(rv, re_, s, N.GM_MODE_CLIQUE)
+
+
+@pytest.mark.parametrize("dmax", [16, 64, 200, 1024])
+def test_bitmap_threshold_does_not_change_counts(dmax):
+    """Roots split between the local-bitmap and list engines at several
+    out-degree bounds (word counts 1, 4 and 16 all exercised)."""
+    g = N.Graph.rmat(13, 24, 6)
+    d = g.orient_by_degree()
+    v, e = d.csr()
+    v, e = v.copy(), e.copy()
+    dg = N.DeviceGraph(v, e)
+    csr = O.CSR(v, e)
+    for k in (3, 4, 5, 6, 7):
+        s = N.Schedule.plain(k, clique_adj(k))
+        s.redundancy = 1
+        desc = s.plan_desc(N.GM_MODE_CLIQUE)
+        raw, st = N.count(dg, N.DevicePlan(desc), flags=N.GM_COUNT_ALG_BYTES, bitmap_max_degree=dmax)
+        assert (raw, st.alg_bytes) == O.count_clique_hybrid(csr, desc, dmax)
+
+
+def test_bitmap_wide_rows():
+    """Out-degrees between 256 and 1024 after orientation: rows of up to 16
+    words held in global scratch."""
+    v, e = random_graph(700, 0.5, 12)
+    d = N.Graph.from_csr(v, e).orient_by_degree()
+    dv, de = d.csr()
+    dv, de = dv.copy(), de.copy()
+    assert np.diff(dv).max() > 256
+    for k in (3, 4):
+        s = N.Schedule.plain(k, clique_adj(k))
+        s.redundancy = 1
+        gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)

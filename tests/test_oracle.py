@@ -215,3 +215,16 @@ def test_reference_miscounts_are_not_inherited(name):
     want = O.brute_force(csr, n, adj)
     assert O.ref_pattern_matching(csr, int(np.diff(v).max()), n, adj) != want
     assert all_rules(csr, n, adj) == (want, want)
+
+
+def test_sampled_cpu_baseline_partitions_the_work():
+    """The CPU baseline's depth-1 sampling: summed over all offsets it is the
+    full count, for any stride."""
+    v, e = random_graph(60, 0.3, 3)
+    csr = O.CSR(v, e)
+    for name in ("house", "k4"):
+        n, adj = PATTERNS[name]
+        s = schedule_for(csr, n, adj)
+        full = O.count_vertex_rooted(csr, s.plan_desc())
+        for stride in (2, 7):
+            assert sum(O.count_vertex_rooted_strided(csr, s.plan_desc(), 4, stride, o) for o in range(stride)) == full
