@@ -81,6 +81,24 @@ def cpu_baseline(N, O, v, e, k, target_s=20.0):
                       f"candidates i = 1 mod {stride}: {t:.1f} s measured, scaled x{stride}"}
 
 
+def calibration_copy(mb):
+    """One device-to-device copy of `mb` MiB (HIP's own copy kernel), so that
+    a PMC run sees a kernel of known traffic next to the counting kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mb << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy: {mb} MiB device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +107,8 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-control", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -98,6 +118,9 @@ def main():
     from graphindor_amd import native as N
     if N.device_count() < 1:
         sys.exit("no GPU visible")
+
+    if args.calib_copy_mb > 0:
+        calibration_copy(args.calib_copy_mb)
 
     t0 = time.time()
     g = N.Graph.rmat(args.scale, 16, 2)

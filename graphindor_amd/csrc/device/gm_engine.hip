@@ -45,6 +45,7 @@ struct DevPlan {
     int32_t size_only[GM_MAX_PREFIX];
     int32_t clique_tail;  // clique mode: depth n-2 counts lane-parallel (see clique_tail_count)
     int32_t clique_chain; // clique mode on the plain k-clique chain: local-bitmap engine eligible
+    int32_t tail_depth;   // pattern mode: depth counted lane-parallel by pattern_tail (-1: none)
     int32_t iep_id[GM_MAX_IEP], iep_vflag[GM_MAX_IEP], iep_vcoef[GM_MAX_IEP];
     int32_t iep_coef[GM_MAX_IEP], iep_flag[GM_MAX_IEP], iep_pos[GM_MAX_IEP];
 };
@@ -66,7 +67,7 @@ struct WaveLds {
 template <int MODE>
 struct WaveShared {
     WaveLds w;
-    uint32_t stage[MODE == GM_MODE_CLIQUE ? kStage : 1];
+    uint32_t stage[kStage];
 };
 
 struct Counters {
@@ -92,6 +93,7 @@ struct KernelArgs {
     Counters* ctr;
     int32_t stats;
     int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
+    int32_t tail_depth;       // pattern mode: depth pattern_tail counts (-1: none)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -381,6 +383,248 @@ __device__ void clique_tail_count(Wave& w, SetRef L) {
     wave_sync();
 }
 
+
+// ---------------------------------------------------------------------------
+// Lane-parallel last candidate level of the pattern rule.
+//
+// At depth D = n-k-1 (IEP tail of k >= 2 vertices) or D = n-2 (no IEP) the
+// sets the schedule builds for a candidate v are only read for their sizes
+// and for membership of the embedded vertices: every prefix q of last[D]
+// is S_q = F_q ∩ N(v) with F_q (its father) fixed above D, or N(v) itself.
+// So each lane takes its own candidate and computes |S_q|, the break-size
+// prune, and then either the IEP terminal of depth D+1
+// (gpu/function/pattern_matching.cuh:110-139) or the restricted last level
+// (pattern_matching.cuh:240-257) for it, without materialising anything.
+// The fathers are staged in LDS when they fit.  Byte model: as the wave
+// path, except that the builds of depth D are charged as size-only.
+// ---------------------------------------------------------------------------
+constexpr int kTailMaxPrefix = 4;
+constexpr int kTailMaxIep = 8;
+
+// |F ∩ N| over the elements below `limit`, iterating the shorter list (per lane).
+__device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
+                                                    uint32_t limit) {
+    int32_t c = 0;
+    if (nF <= nN) {
+        for (int32_t t = 0; t < nF; ++t) {
+            uint32_t x = F[t];
+            if (x >= limit) break;
+            c += contains(Nv, nN, x) ? 1 : 0;
+        }
+    } else {
+        for (int32_t t = 0; t < nN; ++t) {
+            uint32_t x = Nv[t];
+            if (x >= limit) break;
+            c += contains(F, nF, x) ? 1 : 0;
+        }
+    }
+    return c;
+}
+
+template <int D>
+__device__ void pattern_tail(Wave& w, SetRef loop) {
+    const DevPlan& p = *w.pl;
+    const int ln = lane_id();
+    const uint32_t bound = restriction_bound<D>(w);
+    // Prefixes of depth D, their fathers (staged in LDS when they fit).
+    int32_t pid[kTailMaxPrefix];
+    const uint32_t* fptr[kTailMaxPrefix];
+    int32_t flen[kTailMaxPrefix];
+    bool alias[kTailMaxPrefix];
+    int q = p.last[D];
+    int32_t staged = 0;
+#pragma unroll
+    for (int j = 0; j < kTailMaxPrefix; ++j) {
+        pid[j] = q;
+        alias[j] = true;
+        fptr[j] = nullptr;
+        flen[j] = 0;
+        if (q != -1) {
+            int f = p.father[q];
+            if (f >= 0) {
+                SetRef fs = get_set(w, f);
+                alias[j] = false;
+                fptr[j] = fs.ptr;
+                flen[j] = fs.len;
+                staged += fs.len;
+            }
+            q = p.next[q];
+        }
+    }
+    if (staged <= kStage) {
+        int32_t off = 0;
+#pragma unroll
+        for (int j = 0; j < kTailMaxPrefix; ++j) {
+            if (pid[j] == -1 || alias[j]) continue;
+            for (int32_t t = ln; t < flen[j]; t += kWave) w.stage[off + t] = fptr[j][t];
+            fptr[j] = w.stage + off;
+            off += flen[j];
+        }
+        wave_sync();
+    }
+    // IEP rows on sets built above D: |S| - coef, or |S \ emb[0:D)| (uniform).
+    int32_t base[kTailMaxIep];
+    const uint32_t* sptr[kTailMaxIep];
+    int32_t slen[kTailMaxIep];
+    int32_t at[kTailMaxIep];  // position of the row's prefix in pid[], or -1
+#pragma unroll
+    for (int e = 0; e < kTailMaxIep; ++e) {
+        base[e] = 0;
+        sptr[e] = nullptr;
+        slen[e] = 0;
+        at[e] = -1;
+        if (p.k >= 2 && e < p.iep_vn) {
+            int ip = p.iep_id[e];
+#pragma unroll
+            for (int j = 0; j < kTailMaxPrefix; ++j)
+                if (pid[j] == ip) at[e] = j;
+            if (at[e] < 0) {
+                SetRef s = get_set(w, ip);
+                sptr[e] = s.ptr;
+                slen[e] = s.len;
+                base[e] = p.iep_vflag[e] ? s.len - p.iep_vcoef[e] : subtract_emb<D>(w, s.ptr, s.len);
+            }
+        }
+    }
+    // Last loop set when there is no IEP tail.
+    int qlast = -1, qat = -1;
+    SetRef qset;
+    qset.ptr = nullptr;
+    qset.len = 0;
+    if (p.k == 0) {
+        qlast = p.loopset[D + 1];
+#pragma unroll
+        for (int j = 0; j < kTailMaxPrefix; ++j)
+            if (pid[j] == qlast) qat = j;
+        if (qat < 0) qset = get_set(w, qlast);
+    }
+
+    unsigned long long acc = 0, bytes = 0;
+    for (int32_t b0 = 0; b0 < loop.len; b0 += kWave) {
+        const int32_t i = b0 + ln;
+        const bool in_range = i < loop.len;
+        const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
+        const bool live = in_range && v < bound;
+        const bool stop_after = ballot(in_range && v >= bound) != 0;
+        if (live) {
+            bytes += 4ull;
+            if (!in_emb<D>(w, v)) {
+                bytes += 8ull;
+                const int64_t l = w.a.row[v], r = w.a.row[v + 1];
+                const uint32_t* Nv = w.a.col + l;
+                const int32_t dv = (int32_t)(r - l);
+                int32_t sz[kTailMaxPrefix];
+                bool pruned = false;
+#pragma unroll
+                for (int j = 0; j < kTailMaxPrefix; ++j) {
+                    sz[j] = 0;
+                    if (pid[j] == -1 || pruned) continue;
+                    if (alias[j]) {
+                        sz[j] = dv;
+                    } else {
+                        sz[j] = lane_isect_count(fptr[j], flen[j], Nv, dv, 0xffffffffu);
+                        bytes += 4ull * (unsigned long long)(flen[j] + dv);
+                    }
+                    if (sz[j] == p.brk[pid[j]]) pruned = true;
+                }
+                if (!pruned) {
+                    // x in S_j  <=>  x in N(v) and (alias or x in F_j).
+                    auto member = [&](int j, uint32_t x) -> bool {
+                        bool in_n = contains(Nv, dv, x);
+                        bool in_f = true;
+#pragma unroll
+                        for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+                            if (jj == j && !alias[jj]) in_f = contains(fptr[jj], flen[jj], x);
+                        return in_n && in_f;
+                    };
+                    if (p.k >= 2) {
+                        int32_t ans[kTailMaxIep];
+#pragma unroll
+                        for (int e = 0; e < kTailMaxIep; ++e) {
+                            ans[e] = 0;
+                            if (e >= p.iep_vn) continue;
+                            const int j = at[e];
+                            if (j >= 0) {
+                                int32_t szj = 0;
+#pragma unroll
+                                for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+                                    if (jj == j) szj = sz[jj];
+                                if (p.iep_vflag[e]) {
+                                    ans[e] = szj - p.iep_vcoef[e];
+                                } else {
+                                    int32_t c = szj;
+#pragma unroll
+                                    for (int x = 0; x < D; ++x) c -= member(j, w.emb[x]) ? 1 : 0;
+                                    ans[e] = c;
+                                    bytes += 4ull * (D + 1);
+                                }
+                            } else if (p.iep_vflag[e]) {
+                                ans[e] = base[e];
+                            } else {
+                                ans[e] = base[e] - (contains(sptr[e], slen[e], v) ? 1 : 0);
+                                bytes += 4ull * (D + 1);
+                            }
+                        }
+                        int last_pos = -1;
+                        long long val = 0;
+                        for (int pos = 0; pos < p.iep_an; ++pos) {
+                            const int ap = p.iep_pos[pos];
+                            long long x = 0;
+#pragma unroll
+                            for (int e = 0; e < kTailMaxIep; ++e)
+                                if (e == ap) x = ans[e];
+                            if (pos == last_pos + 1) val = x;
+                            else if (val != 0) val = (long long)((unsigned long long)val * (unsigned long long)x);
+                            if (p.iep_flag[pos]) {
+                                last_pos = pos;
+                                acc += (unsigned long long)val * (unsigned long long)(long long)p.iep_coef[pos];
+                            }
+                        }
+                    } else {
+                        // Depth D+1 = n-1: |{x in S : x < bound'} \ emb'| with emb' = emb + v.
+                        uint32_t b1 = 0xffffffffu;
+                        for (int rr = p.rlast[D + 1]; rr != -1; rr = p.rnext[rr]) {
+                            const int idx = p.rindex[rr];
+                            uint32_t ev = (idx == D) ? v : emb_at<D>(w, idx);
+                            b1 = ev < b1 ? ev : b1;
+                        }
+                        int32_t res;
+                        if (qat >= 0) {
+                            bool al = false;
+                            const uint32_t* F = nullptr;
+                            int32_t nF = 0;
+#pragma unroll
+                            for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+                                if (jj == qat) {
+                                    al = alias[jj];
+                                    F = fptr[jj];
+                                    nF = flen[jj];
+                                }
+                            res = al ? lower_bound_u(Nv, dv, b1) : lane_isect_count(F, nF, Nv, dv, b1);
+#pragma unroll
+                            for (int x = 0; x < D; ++x)
+                                if (w.emb[x] < b1 && member(qat, w.emb[x])) --res;
+                        } else {
+                            const int32_t lim = lower_bound_u(qset.ptr, qset.len, b1);
+                            res = lim;
+#pragma unroll
+                            for (int x = 0; x < D; ++x)
+                                if (contains(qset.ptr, lim, w.emb[x])) --res;
+                            if (contains(qset.ptr, lim, v)) --res;
+                        }
+                        acc += (unsigned long long)(long long)res;
+                        bytes += 4ull * (D + 1);
+                    }
+                }
+            }
+        }
+        if (stop_after) break;
+    }
+    w.lane_sum += acc;
+    if (w.a.stats) w.lane_bytes += bytes;
+    wave_sync();
+}
+
 template <int D, int MODE>
 __device__ void dfs(Wave& w) {
     if constexpr (D >= GM_MAX_PATTERN) {
@@ -407,6 +651,10 @@ __device__ void dfs(Wave& w) {
         }
         if (MODE == GM_MODE_CLIQUE && D == n - 2 && p.clique_tail && loop.len <= kStage) {
             clique_tail_count(w, loop);
+            return;
+        }
+        if (MODE == GM_MODE_PATTERN && D == w.a.tail_depth) {
+            pattern_tail<D>(w, loop);
             return;
         }
         const uint32_t bound = (MODE == GM_MODE_PATTERN) ? restriction_bound<D>(w) : 0xffffffffu;
@@ -818,6 +1066,21 @@ DeviceCache& cache_for(int dev) {
     return *caches[dev];
 }
 
+// Depth whose candidates pattern_tail counts lane-parallel, or -1: the last
+// candidate depth (n-k-1 with an IEP tail, n-2 without), at least 2, with at
+// most kTailMaxPrefix prefixes built there and kTailMaxIep IEP rows; not for
+// the vertex-induced rule.  The oracle applies the same rule to its byte model.
+int pattern_tail_depth(const DevPlan& h) {
+    if (h.mode != GM_MODE_PATTERN || h.vinduced) return -1;
+    const int D = h.k >= 2 ? h.n - h.k - 1 : (h.k == 0 ? h.n - 2 : -1);
+    if (D < 2) return -1;
+    int np = 0;
+    for (int q = h.last[D]; q != -1; q = h.next[q]) ++np;
+    if (np > kTailMaxPrefix) return -1;
+    if (h.k >= 2 && h.iep_vn > kTailMaxIep) return -1;
+    return D;
+}
+
 // One 64-lane block per resident slot of the bitmap kernel (its LDS bounds
 // residency at 10 blocks per CU).
 int bm_blocks(int cus) { return cus * 10; }
@@ -1022,6 +1285,7 @@ int gm_plan_create(const gm_plan_desc* d, gm_plan** out) {
             if (h.father[q] >= 0) h.size_only[h.father[q]] = 0;
         for (int dd = 1; dd < n - k; ++dd) h.size_only[h.loopset[dd]] = 0;
     }
+    h.tail_depth = pattern_tail_depth(h);
     if (h.mode == GM_MODE_CLIQUE && n >= 3) {
         // Only the size of the last loop set is read (gpu/gpu_clique.cu:229-232).
         int q = h.loopset[n - 1];
@@ -1142,6 +1406,7 @@ int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, c
         a.ctr = dc.ctr;
         a.stats = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
         a.root_min_outdeg = 0;
+        a.tail_depth = (t && t->lane_tail < 0) ? -1 : p->host.tail_depth;
         const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
         const bool use_bitmap = p->host.mode == GM_MODE_CLIQUE && p->host.clique_chain && bm_max > 0;
         Counters c_bm;

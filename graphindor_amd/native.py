@@ -53,7 +53,7 @@ class GmPlanDesc(ctypes.Structure):
 class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("lane_tail", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class GmStats(ctypes.Structure):
@@ -341,13 +341,14 @@ class DevicePlan:
             pass
 
 
-def count(dgraph, dplan, flags=0, waves_per_cu=0, bitmap_max_degree=0):
+def count(dgraph, dplan, flags=0, waves_per_cu=0, bitmap_max_degree=0, lane_tail=0):
     """Raw sum (before the redundancy division) and the gm_stats, via gm_count."""
     raw = ctypes.c_uint64(0)
     st = GmStats()
     tun = GmTuning()
     tun.waves_per_cu = waves_per_cu
     tun.bitmap_max_degree = bitmap_max_degree
+    tun.lane_tail = lane_tail
     _gm_check(gm().gm_count(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw), ctypes.byref(st)),
               "gm_count")
     return raw.value, st

@@ -94,7 +94,9 @@ typedef struct gm_tuning {
     int32_t lds_set_cap;       /* max elements of a prefix set staged in LDS */
     int32_t bitmap_max_degree; /* GM_MODE_CLIQUE: roots of out-degree up to this use the
                                   local-bitmap engine (0 = default 1024, -1 = off) */
-    int32_t reserved[4];
+    int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted lane-parallel
+                                  (0 = default on, -1 = off: every depth wave-wide) */
+    int32_t reserved[3];
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */

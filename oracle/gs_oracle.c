@@ -70,6 +70,7 @@ typedef struct {
     uint8_t size_only[GM_MAX_PREFIX];
     int gpu_rule;     /* 1: edge-rooted device rule, 0: CPU engine rule */
     int32_t d1_stride, d1_offset; /* CPU rule: expand only depth-1 candidates i = offset mod stride */
+    int tail_depth;   /* device rule: builds at this depth are charged as size-only */
 } or_ctx;
 
 static int64_t max_degree(const or_graph* g) {
@@ -142,6 +143,21 @@ static int32_t subtraction_size(or_ctx* c, const int32_t* set, int32_t limit) {
     return r;
 }
 
+/* Depth whose builds the device counts lane-parallel (pattern_tail in the
+ * engine): the last candidate depth, >= 2, at most 4 prefixes built there,
+ * at most 8 IEP rows, not vertex-induced; -1 otherwise. */
+static int tail_depth_of(const gm_plan_desc* p) {
+    if (p->mode != GM_MODE_PATTERN || p->is_vertex_induced) return -1;
+    int k = p->in_exclusion_optimize_num;
+    int D = k >= 2 ? p->size - k - 1 : (k == 0 ? p->size - 2 : -1);
+    if (D < 2) return -1;
+    int np = 0;
+    for (int q = p->last[D]; q != -1; q = p->next[q]) ++np;
+    if (np > 4) return -1;
+    if (k >= 2 && p->iep_vertex_num > 8) return -1;
+    return D;
+}
+
 static void build(or_ctx* c, int pid, const int32_t* in, int32_t n_in) {
     int f = c->p->father_prefix_id[pid];
     if (f == -1) {
@@ -171,7 +187,8 @@ static void build(or_ctx* c, int pid, const int32_t* in, int32_t n_in) {
     }
     c->sets[pid].ptr = out;
     c->sets[pid].len = k;
-    c->bytes += 4ull * (uint64_t)(na + n_in + k);
+    c->bytes += 4ull * (uint64_t)(na + n_in);
+    if (!(c->gpu_rule && c->emb_n == c->tail_depth)) c->bytes += 4ull * (uint64_t)k;
 }
 
 static void iep_terminal(or_ctx* c) {
@@ -279,6 +296,7 @@ static or_ctx* ctx_new(const or_graph* g, const gm_plan_desc* p, int gpu_rule) {
     c->store = (int32_t*)malloc(sizeof(int32_t) * (size_t)c->cap * (size_t)(p->total_prefix_num + p->size));
     c->sets = (or_set*)calloc((size_t)(p->total_prefix_num + p->size), sizeof(or_set));
     c->gpu_rule = gpu_rule;
+    c->tail_depth = gpu_rule ? tail_depth_of(p) : -1;
     derive_size_only(p, c->size_only);
     return c;
 }
@@ -357,6 +375,7 @@ uint64_t oracle_count_edge_rooted(const or_graph* g, const gm_plan_desc* p, int 
                 uint32_t v1 = (uint32_t)g->edge[e];
                 if (p->restrict_last[1] != -1 && (uint32_t)v0 <= v1) continue;
                 c->bytes += 24ull;
+                c->emb_n = 0; /* root-level builds (depths 0 and 1) */
                 const int32_t* n0 = g->edge + g->vertex[v0];
                 int32_t d0 = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
                 for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, n0, d0);

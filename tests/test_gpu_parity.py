@@ -271,3 +271,21 @@ def test_bitmap_wide_rows():
         s = N.Schedule.plain(k, clique_adj(k))
         s.redundancy = 1
         gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)
+
+
+@pytest.mark.parametrize("name", ["house", "house_cfg", "c5", "k4_pendant", "c6", "prism", "diamond", "tailed_triangle"])
+def test_lane_tail_matches_wave_path(name):
+    """The lane-parallel last candidate level gives the wave-wide path's sum."""
+    n, adj = PATTERNS[name]
+    g = N.Graph.rmat(12, 16, 8)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    for iep in (True, False):
+        s = N.Schedule.build(n, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"], iep=iep)
+        dg = N.DeviceGraph(v, e)
+        dp = N.DevicePlan(s.plan_desc())
+        on, _ = N.count(dg, dp)
+        off, _ = N.count(dg, dp, lane_tail=-1)
+        assert on == off
+        gpu_vs_oracle(v, e, s)
