@@ -1,11 +1,16 @@
 // gpu_graph <graph.g> <pattern_size> <pattern_adj_string> [1|0 iep]
 // Same arguments and stdout lines as the reference (gpu/gpu_graph.cu:53-114).
+// With GRAPHSET_COST_MODEL=<file> the schedule is the learned cost model's
+// pick among the analytic selector's candidates (the analytic pick if the
+// file is unusable); the choice is reported on stderr.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
+#include <memory>
 
 #include "cli_common.h"
+#include "costmodel.h"
 #include "dataloader.h"
 #include "graph.h"
 #include "pattern.h"
@@ -51,11 +56,23 @@ int main(int argc, char* argv[]) {
     tmpTime.check();
     try {
         bool is_pattern_valid = true;
-        Schedule_IEP schedule_iep(p, is_pattern_valid, 1, 1, enable_iep != 0, g->v_cnt, g->e_cnt, g->tri_cnt);
+        std::unique_ptr<Schedule_IEP> owned;
+        const char* model_path = getenv("GRAPHSET_COST_MODEL");
+        if (model_path && *model_path) {
+            std::unique_ptr<CostModel> model = CostModel::load(model_path);
+            if (!model) fprintf(stderr, "cost model %s unusable: analytic schedule\n", model_path);
+            ScheduleChoice ch;
+            owned.reset(select_schedule(p, graph_stats(*g), model.get(), enable_iep != 0, &ch));
+            fprintf(stderr, "Schedule: %s pick, candidate %d of %d (analytic pick %d), %.3f ms\n",
+                    ch.learned ? "learned" : "analytic", ch.chosen, ch.candidates, ch.analytic, ch.select_ms);
+        } else {
+            owned.reset(new Schedule_IEP(p, is_pattern_valid, 1, 1, enable_iep != 0, g->v_cnt, g->e_cnt, g->tri_cnt));
+        }
         if (!is_pattern_valid) {
             fprintf(stderr, "pattern is invalid!\n");
             return 1;
         }
+        const Schedule_IEP& schedule_iep = *owned;
         tmpTime.print("Schedule time cost");
         tmpTime.check();
         g->device_graph();

@@ -71,8 +71,8 @@ struct WaveShared {
 };
 
 struct Counters {
-    unsigned long long sum;
-    unsigned long long cursor;
+    unsigned long long bm_cursor;  // local-bitmap engine's root cursor
+    unsigned long long cursor;     // enumerate kernel's task cursor
     unsigned long long bytes;
     unsigned long long tasks;
     unsigned int overflow;
@@ -84,7 +84,16 @@ struct KernelArgs {
     const uint32_t* col;
     int32_t nv;
     int64_t nnz;
-    const DevPlan* plan;
+    const DevPlan* const* plans;  // np plans counted in one launch
+    int32_t np;
+    const int64_t* gedge;         // ng+1 root-edge offsets: graph g owns edges [gedge[g], gedge[g+1])
+    int32_t ng;
+    unsigned long long* sums;     // [ng * np] raw sums
+    int32_t tail_off;             // 1: never use the lane-parallel pattern tail
+    int32_t chunk_mult;           // root edges per task = 64 * chunk_mult
+    int64_t root_begin, root_end; // root edges counted: [gedge[0], gedge[ng])
+    int64_t root_stride;          // counted root edges: root_begin + i * root_stride < root_end
+    int64_t nroots;               // number of counted root edges
     uint32_t* scratch;
     int64_t wave_stride;  // uint32 per wave
     int32_t cap;          // capacity of a prefix slot
@@ -93,7 +102,6 @@ struct KernelArgs {
     Counters* ctr;
     int32_t stats;
     int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
-    int32_t tail_depth;       // pattern mode: depth pattern_tail counts (-1: none)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -151,6 +159,7 @@ __device__ __forceinline__ int32_t lower_bound_u(const uint32_t* b, int32_t nb, 
 struct Wave {
     KernelArgs a;
     const DevPlan* pl;
+    int tail_depth;
     WaveLds* lds;
     uint32_t* stage;
     uint32_t* scratch;
@@ -653,7 +662,7 @@ __device__ void dfs(Wave& w) {
             clique_tail_count(w, loop);
             return;
         }
-        if (MODE == GM_MODE_PATTERN && D == w.a.tail_depth) {
+        if (MODE == GM_MODE_PATTERN && D == w.tail_depth) {
             pattern_tail<D>(w, loop);
             return;
         }
@@ -730,12 +739,33 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
     return x;
 }
 
+// Graph of root edge e: largest g with gedge[g] <= e (uniform trip count).
+__device__ __forceinline__ int32_t edge_graph(const int64_t* gedge, int32_t ng, int64_t e) {
+    int32_t base = 0, n = ng;
+    while (n > 1) {
+        int32_t half = n >> 1;
+        base = (gedge[base + half] <= e) ? base + half : base;
+        n -= half;
+    }
+    return base;
+}
+
+// Adds the wave's partial sum for (graph, plan) to sums[] and clears it.
+__device__ __forceinline__ void flush_sum(Wave& w, int32_t g, int32_t pi) {
+    const unsigned long long total = w.sum + wave_sum(w.lane_sum);
+    if (lane_id() == 0 && total != 0ull) atomicAdd(&w.a.sums[(int64_t)g * w.a.np + pi], total);
+    w.sum = 0;
+    w.lane_sum = 0;
+}
+
+// Persistent enumeration over tasks (root chunk c, plan pi), chunk-major so
+// the plans of one chunk run close together and share its neighbour lists
+// in cache.  A root's count goes to the (graph, plan) cell of its edge.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     __shared__ WaveShared<MODE> lds_all[kWavesPerBlock];
     Wave w;
     w.a = a;
-    w.pl = a.plan;
     const int wid = threadIdx.x / kWave;
     w.lds = &lds_all[wid].w;
     w.stage = lds_all[wid].stage;
@@ -754,43 +784,74 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
         w.lds->sets[i].len = 0;
     }
     wave_sync();
-    const DevPlan& p = *w.pl;
-    const bool restricted1 = p.rlast[1] != -1;
     const int ln = lane_id();
-    unsigned long long tasks = 0;
+    const int64_t task_edges = (int64_t)kWave * a.chunk_mult;
+    unsigned long long roots = 0;
+    int32_t cur_g = 0, cur_p = 0;
     while (true) {
-        unsigned long long chunk = 0;
-        if (ln == 0) chunk = atomicAdd(&a.ctr->cursor, 1ull);
-        chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
-        if ((long long)chunk >= a.nchunks) break;
-        int64_t e = (int64_t)chunk * kWave + ln;
-        bool ok = e < a.nnz;
-        uint32_t v0 = 0, v1 = 0;
-        if (ok) {
-            v1 = a.col[e];
-            v0 = (uint32_t)edge_source(a.row, a.nv, e);
-            if (restricted1 && v0 <= v1) ok = false;
-            if (MODE == GM_MODE_CLIQUE && a.root_min_outdeg > 0 && a.row[v0 + 1] - a.row[v0] <= a.root_min_outdeg)
-                ok = false;
-        }
-        uint64_t m = ballot(ok);
-        while (m != 0) {
-            int j = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1;
-            uint32_t u0 = rdlane(v0, j), u1 = rdlane(v1, j);
-            ++tasks;
-            if (a.stats) w.bytes += 24ull;
-            process_root<MODE>(w, u0, u1);
+        unsigned long long task = 0;
+        if (ln == 0) task = atomicAdd(&a.ctr->cursor, 1ull);
+        task = (unsigned long long)uni64((int64_t)__shfl((long long)task, 0));
+        if ((long long)task >= a.nchunks) break;
+        const int32_t pi = (int32_t)(task % (unsigned long long)a.np);
+        const int64_t chunk = (int64_t)(task / (unsigned long long)a.np);
+        w.pl = a.plans[pi];
+        const DevPlan& p = *w.pl;
+        w.tail_depth = a.tail_off ? -1 : p.tail_depth;
+        const bool restricted1 = p.rlast[1] != -1;
+        // Logical root indices [i_first, i_end) map to edges root_begin + i * root_stride.
+        const int64_t i_first = chunk * task_edges;
+        const int64_t i_end = i_first + task_edges < a.nroots ? i_first + task_edges : a.nroots;
+        for (int64_t i0 = i_first; i0 < i_end; i0 += kWave) {
+            const int64_t e = a.root_begin + (i0 + ln) * a.root_stride;
+            bool ok = i0 + ln < i_end;
+            uint32_t v0 = 0, v1 = 0;
+            int32_t g = 0;
+            if (ok) {
+                v1 = a.col[e];
+                v0 = (uint32_t)edge_source(a.row, a.nv, e);
+                if (a.ng > 1) g = edge_graph(a.gedge, a.ng, e);
+                if (restricted1 && v0 <= v1) ok = false;
+                if (MODE == GM_MODE_CLIQUE && a.root_min_outdeg > 0 && a.row[v0 + 1] - a.row[v0] <= a.root_min_outdeg)
+                    ok = false;
+            }
+            uint64_t m = ballot(ok);
+            while (m != 0) {
+                int j = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const uint32_t u0 = rdlane(v0, j), u1 = rdlane(v1, j);
+                const int32_t gj = (int32_t)rdlane((uint32_t)g, j);
+                if (gj != cur_g || pi != cur_p) {
+                    flush_sum(w, cur_g, cur_p);
+                    cur_g = gj;
+                    cur_p = pi;
+                }
+                ++roots;
+                if (a.stats) w.bytes += 24ull;
+                process_root<MODE>(w, u0, u1);
+            }
         }
     }
-    const unsigned long long lane_total = wave_sum(w.lane_sum);
+    flush_sum(w, cur_g, cur_p);
     const unsigned long long lane_bytes = a.stats ? wave_sum(w.lane_bytes) : 0ull;
     if (ln == 0) {
-        atomicAdd(&a.ctr->sum, w.sum + lane_total);
-        atomicAdd(&a.ctr->tasks, tasks);
+        atomicAdd(&a.ctr->tasks, roots);
         if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes + lane_bytes);
         if (w.overflow) atomicOr(&a.ctr->overflow, 1u);
     }
+}
+
+// Adds a clique launch's counters into the call's totals.
+__global__ void accumulate_counters_kernel(Counters* total, const Counters* part) {
+    total->bytes += part->bytes;
+    total->tasks += part->tasks;
+    total->overflow |= part->overflow;
+}
+
+// Zeroes n 64-bit words (a kernel, so that it may be captured in a graph).
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 
@@ -825,6 +886,7 @@ struct BmArgs {
     int32_t max_d;             // roots with out-degree above this are left to the list engine
     unsigned long long* rows_scratch;  // per wave kBmMaxD * kBmMaxW words
     int64_t nchunks;           // root chunks of kBmChunk vertices
+    unsigned long long* out;   // the plan's sum cell
     Counters* ctr;
     int32_t stats;
 };
@@ -1001,7 +1063,7 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
     unsigned long long roots = 0;
     while (true) {
         unsigned long long chunk = 0;
-        if (ln == 0) chunk = atomicAdd(&a.ctr->cursor, 1ull);
+        if (ln == 0) chunk = atomicAdd(&a.ctr->bm_cursor, 1ull);
         chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
         if ((long long)chunk >= a.nchunks) break;
         const int32_t vbeg = (int32_t)(chunk * kBmChunk);
@@ -1020,7 +1082,7 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
     const unsigned long long total = wave_sum(acc);
     const unsigned long long tb = a.stats ? wave_sum(bytes) : 0ull;
     if (ln == 0) {
-        atomicAdd(&a.ctr->sum, total);
+        atomicAdd(a.out, total);
         atomicAdd(&a.ctr->tasks, roots);
         if (a.stats) atomicAdd(&a.ctr->bytes, tb + wave_bytes);
     }
@@ -1053,6 +1115,12 @@ struct DeviceCache {
     size_t scratch_bytes = 0;
     void* bm_rows = nullptr;
     size_t bm_bytes = 0;
+    void* plan_ptrs = nullptr;
+    size_t plan_ptrs_bytes = 0;
+    void* gedge = nullptr;
+    size_t gedge_bytes = 0;
+    void* sums = nullptr;
+    size_t sums_bytes = 0;
     Counters* ctr = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -1108,6 +1176,8 @@ struct gm_graph {
     int32_t max_deg;
     int32_t second_deg;
 };
+
+struct gm_batch;
 
 struct gm_plan {
     DevPlan host;
@@ -1340,19 +1410,20 @@ void gm_plan_destroy(gm_plan* p) {
     delete p;
 }
 
-int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, const gm_tuning* t, int32_t flags,
-                   void* stream, uint64_t* raw_sums, gm_stats* stats);
-
-int gm_count(const gm_graph* g, const gm_plan* p, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sum,
-             gm_stats* stats) {
-    return gm_count_multi(g, &p, 1, t, flags, stream, raw_sum, stats);
-}
-
-int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, const gm_tuning* t, int32_t flags,
-                   void* stream, uint64_t* raw_sums, gm_stats* stats) {
-    if (!g || !plans || np <= 0 || !raw_sums) return fail(GM_ERR_INVALID, "gm_count: bad arguments");
+int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng, const gm_plan* const* plans,
+                   int32_t np, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sums, gm_stats* stats) {
+    if (!g || !plans || np <= 0 || !raw_sums || ng < 1) return fail(GM_ERR_INVALID, "gm_count: bad arguments");
     for (int i = 0; i < np; ++i)
         if (!plans[i]) return fail(GM_ERR_INVALID, "gm_count: null plan");
+    if (graph_edge_offsets) {
+        if (graph_edge_offsets[0] < 0 || graph_edge_offsets[ng] > g->nnz)
+            return fail(GM_ERR_INVALID, "gm_count_batch: edge offsets outside [0, nnz]");
+        for (int i = 0; i < ng; ++i)
+            if (graph_edge_offsets[i + 1] < graph_edge_offsets[i])
+                return fail(GM_ERR_INVALID, "gm_count_batch: edge offsets not monotone");
+    } else if (ng > 1) {
+        return fail(GM_ERR_INVALID, "gm_count_batch: edge offsets required for more than one graph");
+    }
     HIP_TRY(hipSetDevice(g->device));
     hipStream_t s = (hipStream_t)stream;
     DeviceCache& dc = cache_for(g->device);
@@ -1360,109 +1431,336 @@ int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, c
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
     // Default grid: every block the occupancy of either kernel allows
-    // (persistent waves; the kernels exit when the root cursor runs out).
+    // (persistent waves; the kernels exit when the task cursor runs out).
     int occ_pattern = 0, occ_clique = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pattern, enumerate_kernel<GM_MODE_PATTERN>, kBlock, 0));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_clique, enumerate_kernel<GM_MODE_CLIQUE>, kBlock, 0));
-    bool any_clique = false;
-    for (int i = 0; i < np; ++i) any_clique |= plans[i]->host.mode == GM_MODE_CLIQUE;
-    int max_blocks_per_cu = std::max(1, any_clique ? std::min(occ_pattern, occ_clique) : occ_pattern);
-    int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : max_blocks_per_cu * kWavesPerBlock;
-    int blocks = std::max(1, cus * std::min(waves_per_cu, max_blocks_per_cu * kWavesPerBlock) / kWavesPerBlock);
+    const int max_blocks_per_cu = std::max(1, std::min(occ_pattern, occ_clique));
+    const int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : max_blocks_per_cu * kWavesPerBlock;
+    const int blocks = std::max(1, cus * std::min(waves_per_cu, max_blocks_per_cu * kWavesPerBlock) / kWavesPerBlock);
     const int64_t nwaves = (int64_t)blocks * kWavesPerBlock;
     const int32_t cap = std::max(1, g->second_deg);
     const int32_t cap_vi = std::max(1, g->max_deg);
-    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters)));
+    const int chunk_mult = (t && t->chunk_edges > 0) ? std::max(1, t->chunk_edges / kWave) : ((ng > 1 || np > 1) ? 4 : 1);
+    const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
+    const int stats_on = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
+    const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
+
+    // Plans in launch order: pattern plans first (one fused launch), then
+    // clique plans.
+    std::vector<int> order;
+    for (int i = 0; i < np; ++i)
+        if (plans[i]->host.mode == GM_MODE_PATTERN) order.push_back(i);
+    const int np_pat = (int)order.size();
+    if (root_stride > 1 && np_pat != np) return fail(GM_ERR_INVALID, "gm_count: root_stride applies to GM_MODE_PATTERN plans only");
+    for (int i = 0; i < np; ++i)
+        if (plans[i]->host.mode == GM_MODE_CLIQUE) order.push_back(i);
+    const int np_clq = np - np_pat;
+
+    int64_t stride = 1;
+    for (int i = 0; i < np; ++i) {
+        const DevPlan& h = plans[i]->host;
+        stride = std::max<int64_t>(stride, (int64_t)h.P * cap + (int64_t)(h.vinduced ? h.n : 0) * cap_vi);
+    }
+    auto grow = [&](void** buf, size_t* have, size_t need) -> hipError_t {
+        if (need <= *have) return hipSuccess;
+        if (*buf) {
+            hipError_t e = hipFree(*buf);
+            if (e != hipSuccess) return e;
+        }
+        *buf = nullptr;
+        *have = 0;
+        hipError_t e = hipMalloc(buf, need);
+        if (e == hipSuccess) *have = need;
+        return e;
+    };
+    HIP_TRY(grow(&dc.scratch, &dc.scratch_bytes, (size_t)stride * (size_t)nwaves * sizeof(uint32_t)));
+    HIP_TRY(grow(&dc.plan_ptrs, &dc.plan_ptrs_bytes, sizeof(DevPlan*) * (size_t)np));
+    HIP_TRY(grow(&dc.gedge, &dc.gedge_bytes, sizeof(int64_t) * (size_t)(ng + 1)));
+    HIP_TRY(grow(&dc.sums, &dc.sums_bytes, sizeof(unsigned long long) * (size_t)ng * (size_t)np));
+    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * 2));
     if (!dc.ev0) {
         HIP_TRY(hipEventCreate(&dc.ev0));
         HIP_TRY(hipEventCreate(&dc.ev1));
     }
-    double total_ms = 0;
-    unsigned long long total_bytes = 0, total_tasks = 0;
-    for (int pi = 0; pi < np; ++pi) {
-        const gm_plan* p = plans[pi];
-        const int n = p->host.n;
-        const int64_t stride = (int64_t)p->host.P * cap + (int64_t)(p->host.vinduced ? n : 0) * cap_vi;
-        const size_t need = (size_t)std::max<int64_t>(stride, 1) * (size_t)nwaves * sizeof(uint32_t);
-        if (need > dc.scratch_bytes) {
-            if (dc.scratch) HIP_TRY(hipFree(dc.scratch));
-            dc.scratch = nullptr;
-            dc.scratch_bytes = 0;
-            HIP_TRY(hipMalloc(&dc.scratch, need));
-            dc.scratch_bytes = need;
-        }
-        HIP_TRY(hipMemsetAsync(dc.ctr, 0, sizeof(Counters), s));
-        KernelArgs a;
-        a.row = g->d_row;
-        a.col = g->d_col;
-        a.nv = g->n;
-        a.nnz = g->nnz;
-        a.plan = p->d_plan;
-        a.scratch = (uint32_t*)dc.scratch;
-        a.wave_stride = stride;
-        a.cap = cap;
-        a.cap_vi = cap_vi;
-        a.nchunks = (g->nnz + kWave - 1) / kWave;
-        a.ctr = dc.ctr;
-        a.stats = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
+    std::vector<const DevPlan*> hptrs(np);
+    for (int k = 0; k < np; ++k) hptrs[k] = plans[order[k]]->d_plan;
+    HIP_TRY(hipMemcpyAsync(dc.plan_ptrs, hptrs.data(), sizeof(DevPlan*) * np, hipMemcpyHostToDevice, s));
+    std::vector<int64_t> hgedge(2, 0);
+    if (graph_edge_offsets) hgedge.assign(graph_edge_offsets, graph_edge_offsets + ng + 1);
+    else hgedge[1] = g->nnz;
+    HIP_TRY(hipMemcpyAsync(dc.gedge, hgedge.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, s));
+    unsigned long long* d_sums = (unsigned long long*)dc.sums;
+    const DevPlan* const* d_plans = (const DevPlan* const*)dc.plan_ptrs;
+    const int64_t ncells = (int64_t)ng * np;
+    const int64_t nctr_words = (int64_t)(sizeof(Counters) * 2 / sizeof(unsigned long long));
+
+    HIP_TRY(hipEventRecord(dc.ev0, s));
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, s, d_sums, ncells);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)dc.ctr, nctr_words);
+    HIP_TRY(hipGetLastError());
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.row = g->d_row;
+    a.col = g->d_col;
+    a.nv = g->n;
+    a.nnz = g->nnz;
+    a.gedge = (const int64_t*)dc.gedge;
+    a.ng = ng;
+    a.scratch = (uint32_t*)dc.scratch;
+    a.wave_stride = stride;
+    a.cap = cap;
+    a.cap_vi = cap_vi;
+    a.ctr = dc.ctr;
+    a.stats = stats_on;
+    a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
+    a.chunk_mult = chunk_mult;
+    a.root_begin = hgedge[0];
+    a.root_end = hgedge[ng];
+    a.root_stride = root_stride;
+    a.nroots = (a.root_end - a.root_begin + root_stride - 1) / root_stride;
+    const int64_t chunks = (a.nroots + (int64_t)kWave * chunk_mult - 1) / ((int64_t)kWave * chunk_mult);
+    if (np_pat > 0) {
+        a.plans = d_plans;
+        a.np = np_pat;
+        a.sums = d_sums;
+        a.nchunks = chunks * np_pat;
         a.root_min_outdeg = 0;
-        a.tail_depth = (t && t->lane_tail < 0) ? -1 : p->host.tail_depth;
-        const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
-        const bool use_bitmap = p->host.mode == GM_MODE_CLIQUE && p->host.clique_chain && bm_max > 0;
-        Counters c_bm;
-        std::memset(&c_bm, 0, sizeof(c_bm));
-        HIP_TRY(hipEventRecord(dc.ev0, s));
+        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, s, a);
+        HIP_TRY(hipGetLastError());
+    }
+    // Clique plans: one at a time; on a single graph the local-bitmap engine
+    // takes the roots of out-degree <= bm_max first.
+    Counters* ctr2 = dc.ctr + 1;
+    for (int c = 0; c < np_clq; ++c) {
+        const gm_plan* p = plans[order[np_pat + c]];
+        unsigned long long* cell = d_sums + (int64_t)ng * np_pat + c;  // stride 1 per graph below
+        const bool use_bitmap = ng == 1 && !graph_edge_offsets && p->host.clique_chain && bm_max > 0;
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr2,
+                           (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
+        a.ctr = ctr2;
+        a.root_min_outdeg = 0;
         if (use_bitmap) {
-            // Roots of out-degree <= kBmMaxD: local-bitmap engine; the rest:
-            // the list engine below.
-            const size_t bm_need = (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long);
-            if (bm_need > dc.bm_bytes) {
-                if (dc.bm_rows) HIP_TRY(hipFree(dc.bm_rows));
-                dc.bm_rows = nullptr;
-                dc.bm_bytes = 0;
-                HIP_TRY(hipMalloc(&dc.bm_rows, bm_need));
-                dc.bm_bytes = bm_need;
-            }
+            HIP_TRY(grow(&dc.bm_rows, &dc.bm_bytes,
+                         (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long)));
             BmArgs b;
             b.row = g->d_row;
             b.col = g->d_col;
             b.nv = g->n;
-            b.k = n;
+            b.k = p->host.n;
             b.max_d = bm_max;
             b.rows_scratch = (unsigned long long*)dc.bm_rows;
             b.nchunks = ((int64_t)g->n + kBmChunk - 1) / kBmChunk;
-            b.ctr = dc.ctr;
-            b.stats = a.stats;
-            HIP_TRY(launch_bitmap(n, bm_blocks(cus), b, s));
-            HIP_TRY(hipMemcpyAsync(&c_bm, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemsetAsync(dc.ctr, 0, sizeof(Counters), s));
+            b.out = cell;
+            b.ctr = ctr2;
+            b.stats = stats_on;
+            HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
             a.root_min_outdeg = bm_max;
         }
-        if (p->host.mode == GM_MODE_CLIQUE)
-            hipLaunchKernelGGL(enumerate_kernel<GM_MODE_CLIQUE>, dim3(blocks), dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, s, a);
+        // Per-graph cells of this plan are ng apart: give the kernel the
+        // column as its own 1-plan table.
+        a.plans = d_plans + np_pat + c;
+        a.np = 1;
+        a.sums = cell;
+        a.nchunks = chunks;
+        if (ng > 1) {
+            // cells for graph g at cell[g * np_clq]: use a private region instead
+            a.sums = d_sums + (int64_t)ng * np_pat + (int64_t)c * ng;
+        }
+        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_CLIQUE>, dim3(blocks), dim3(kBlock), 0, s, a);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(dc.ev1, s));
-        Counters c;
-        HIP_TRY(hipMemcpyAsync(&c, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
-        if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
-        raw_sums[pi] = c.sum + c_bm.sum;
-        total_ms += ms;
-        total_bytes += c.bytes + c_bm.bytes;
-        total_tasks += c.tasks + c_bm.tasks;
+        hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr2);
+        a.ctr = dc.ctr;
+    }
+    HIP_TRY(hipEventRecord(dc.ev1, s));
+    std::vector<unsigned long long> hs(ncells);
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(hs.data(), d_sums, sizeof(unsigned long long) * ncells, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&c, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
+    if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
+    // Scatter back to raw_sums[graph * np + plan] in the caller's plan order.
+    for (int gi = 0; gi < ng; ++gi) {
+        for (int k = 0; k < np_pat; ++k) raw_sums[(int64_t)gi * np + order[k]] = hs[(int64_t)gi * np_pat + k];
+        for (int cc = 0; cc < np_clq; ++cc)
+            raw_sums[(int64_t)gi * np + order[np_pat + cc]] =
+                ng > 1 ? hs[(int64_t)ng * np_pat + (int64_t)cc * ng + gi] : hs[(int64_t)ng * np_pat + cc];
     }
     if (stats) {
-        stats->kernel_ms = total_ms;
-        stats->alg_bytes = total_bytes;
-        stats->root_tasks = total_tasks;
+        stats->kernel_ms = ms;
+        stats->alg_bytes = c.bytes;
+        stats->root_tasks = c.tasks;
         stats->grid_waves = (int32_t)nwaves;
         stats->flags_used = flags;
     }
     return GM_OK;
+}
+
+int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, const gm_tuning* t, int32_t flags,
+                   void* stream, uint64_t* raw_sums, gm_stats* stats) {
+    return gm_count_batch(g, nullptr, 1, plans, np, t, flags, stream, raw_sums, stats);
+}
+
+struct gm_batch {
+    int device = 0;
+    int32_t ng = 0, np = 0;
+    int64_t nwaves = 0;
+    int stats = 0;
+    void* d_plans = nullptr;
+    void* d_gedge = nullptr;
+    void* d_sums = nullptr;
+    void* d_ctr = nullptr;
+    void* d_scratch = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+void gm_batch_destroy(gm_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (void* q : {b->d_plans, b->d_gedge, b->d_sums, b->d_ctr, b->d_scratch})
+        if (q) (void)hipFree(q);
+    delete b;
+}
+
+int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng, const gm_plan* const* plans,
+                    int32_t np, const gm_tuning* t, int32_t flags, gm_batch** out) {
+    if (!out) return fail(GM_ERR_INVALID, "gm_batch_create: null output");
+    *out = nullptr;
+    if (!g || !plans || np <= 0 || ng < 1) return fail(GM_ERR_INVALID, "gm_batch_create: bad arguments");
+    if (ng > 1 && !graph_edge_offsets) return fail(GM_ERR_INVALID, "gm_batch_create: edge offsets required");
+    for (int i = 0; i < np; ++i) {
+        if (!plans[i]) return fail(GM_ERR_INVALID, "gm_batch_create: null plan");
+        if (plans[i]->host.mode != GM_MODE_PATTERN)
+            return fail(GM_ERR_UNSUPPORTED, "gm_batch_create: only pattern-rule plans are batched");
+    }
+    std::vector<int64_t> hgedge(2, 0);
+    if (graph_edge_offsets) {
+        hgedge.assign(graph_edge_offsets, graph_edge_offsets + ng + 1);
+        if (hgedge[0] < 0 || hgedge[ng] > g->nnz) return fail(GM_ERR_INVALID, "gm_batch_create: edge offsets outside [0, nnz]");
+        for (int i = 0; i < ng; ++i)
+            if (hgedge[i + 1] < hgedge[i]) return fail(GM_ERR_INVALID, "gm_batch_create: edge offsets not monotone");
+    } else {
+        hgedge[1] = g->nnz;
+    }
+    HIP_TRY(hipSetDevice(g->device));
+    gm_batch* b = new gm_batch();
+    b->device = g->device;
+    b->ng = ng;
+    b->np = np;
+    b->stats = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
+    auto bail = [&](int code, const std::string& msg) {
+        gm_batch_destroy(b);
+        return fail(code, msg);
+    };
+    int cus = 0, occ = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, enumerate_kernel<GM_MODE_PATTERN>, kBlock, 0) != hipSuccess)
+        return bail(GM_ERR_HIP, "gm_batch_create: device query failed");
+    occ = std::max(1, occ);
+    const int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : occ * kWavesPerBlock;
+    const int blocks = std::max(1, cus * std::min(waves_per_cu, occ * kWavesPerBlock) / kWavesPerBlock);
+    b->nwaves = (int64_t)blocks * kWavesPerBlock;
+    const int32_t cap = std::max(1, g->second_deg), cap_vi = std::max(1, g->max_deg);
+    int64_t stride = 1;
+    for (int i = 0; i < np; ++i) {
+        const DevPlan& h = plans[i]->host;
+        stride = std::max<int64_t>(stride, (int64_t)h.P * cap + (int64_t)(h.vinduced ? h.n : 0) * cap_vi);
+    }
+    const int64_t ncells = (int64_t)ng * np;
+    if (hipMalloc(&b->d_plans, sizeof(DevPlan*) * np) != hipSuccess ||
+        hipMalloc(&b->d_gedge, sizeof(int64_t) * (ng + 1)) != hipSuccess ||
+        hipMalloc(&b->d_sums, sizeof(unsigned long long) * ncells) != hipSuccess ||
+        hipMalloc(&b->d_ctr, sizeof(Counters)) != hipSuccess ||
+        hipMalloc(&b->d_scratch, (size_t)stride * (size_t)b->nwaves * sizeof(uint32_t)) != hipSuccess)
+        return bail(GM_ERR_OOM, "gm_batch_create: device allocation failed");
+    std::vector<const DevPlan*> hptrs(np);
+    for (int i = 0; i < np; ++i) hptrs[i] = plans[i]->d_plan;
+    if (hipMemcpy(b->d_plans, hptrs.data(), sizeof(DevPlan*) * np, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_gedge, hgedge.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(GM_ERR_HIP, "gm_batch_create: upload failed");
+    KernelArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.row = g->d_row;
+    a.col = g->d_col;
+    a.nv = g->n;
+    a.nnz = g->nnz;
+    a.plans = (const DevPlan* const*)b->d_plans;
+    a.np = np;
+    a.gedge = (const int64_t*)b->d_gedge;
+    a.ng = ng;
+    a.sums = (unsigned long long*)b->d_sums;
+    a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
+    a.chunk_mult = (t && t->chunk_edges > 0) ? std::max(1, t->chunk_edges / kWave) : 4;
+    a.root_begin = hgedge[0];
+    a.root_end = hgedge[ng];
+    a.scratch = (uint32_t*)b->d_scratch;
+    a.wave_stride = stride;
+    a.cap = cap;
+    a.cap_vi = cap_vi;
+    a.ctr = (Counters*)b->d_ctr;
+    a.stats = b->stats;
+    a.root_stride = 1;
+    a.nroots = a.root_end - a.root_begin;
+    a.nchunks = ((a.nroots + (int64_t)kWave * a.chunk_mult - 1) / ((int64_t)kWave * a.chunk_mult)) * np;
+    // Capture: zero the cells and counters with kernels (no memset nodes),
+    // then the fused launch.
+    hipStream_t cap_stream = nullptr;
+    if (hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(GM_ERR_HIP, "gm_batch_create: stream creation failed");
+    hipError_t e = hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, cap_stream, (unsigned long long*)b->d_sums, ncells);
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, cap_stream, (unsigned long long*)b->d_ctr,
+                           (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
+        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, cap_stream, a);
+        hipError_t e2 = hipStreamEndCapture(cap_stream, &b->graph);
+        e = (e2 != hipSuccess) ? e2 : hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap_stream);
+    if (e != hipSuccess) return bail(GM_ERR_HIP, std::string("gm_batch_create: graph capture failed: ") + hipGetErrorString(e));
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+        return bail(GM_ERR_HIP, "gm_batch_create: event creation failed");
+    *out = b;
+    return GM_OK;
+}
+
+int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats) {
+    if (!b || !raw_sums) return fail(GM_ERR_INVALID, "gm_batch_run: bad arguments");
+    HIP_TRY(hipSetDevice(b->device));
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t ncells = (int64_t)b->ng * b->np;
+    HIP_TRY(hipEventRecord(b->ev0, s));
+    HIP_TRY(hipGraphLaunch(b->exec, s));
+    HIP_TRY(hipEventRecord(b->ev1, s));
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(raw_sums, b->d_sums, sizeof(unsigned long long) * ncells, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&c, b->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (c.overflow) return fail(GM_ERR_INVALID, "gm_batch_run: an intersection exceeded its scratch slot");
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->alg_bytes = c.bytes;
+        stats->root_tasks = c.tasks;
+        stats->grid_waves = (int32_t)b->nwaves;
+        stats->flags_used = b->stats;
+    }
+    return GM_OK;
+}
+
+int gm_count(const gm_graph* g, const gm_plan* p, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sum,
+             gm_stats* stats) {
+    return gm_count_batch(g, nullptr, 1, &p, 1, t, flags, stream, raw_sum, stats);
 }
 
 }  // extern "C"

@@ -5,8 +5,12 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <stdexcept>
 #include <string>
 
+#include "autotune.h"
+#include "costmodel.h"
 #include "dataloader.h"
 #include "generator.h"
 #include "gpu_plan.h"
@@ -81,6 +85,11 @@ GS_API void* gs_graph_er(int32_t n, double p, uint64_t seed) {
     return generate_erdos_renyi(n, p, seed, &mi);
 }
 
+GS_API void* gs_graph_batch(int count, uint64_t seed_base, double log2_min, double log2_max, double deg_min,
+                            double deg_max, int64_t* vertex_off, int64_t* edge_off) {
+    return generate_batch(count, seed_base, log2_min, log2_max, deg_min, deg_max, vertex_off, edge_off);
+}
+
 GS_API void gs_graph_free(void* g) { delete (Graph*)g; }
 
 // out: v_cnt, e_cnt, tri_cnt, second-largest degree
@@ -138,6 +147,57 @@ GS_API void* gs_schedule_new_plain(int n, const char* adj) {
     guarded([&] {
         Pattern p(n, adj);
         out = new GsSchedule{new Schedule_IEP(p.get_adj_mat_ptr(), n), FlatPlan(), -1};
+        return 0;
+    });
+    return out;
+}
+
+// Candidates of the analytic selector.  Record per candidate in `out`:
+// use_iep, npairs, order[n], pairs[2 * npairs]; costs[i] = analytic cost.
+// Returns the number of candidates, or -1 (error) / -2 (buffer too small).
+GS_API int gs_schedule_candidates(int n, const char* adj, int iep, int32_t v_cnt, int64_t e_cnt, int64_t tri_cnt,
+                                  int32_t* out, int cap, double* costs, int cost_cap) {
+    int rc = -1;
+    guarded([&] {
+        Pattern p(n, adj);
+        std::vector<ScheduleCandidate> cs = Schedule_IEP::enumerate_candidates(p, iep != 0, v_cnt, e_cnt, tri_cnt);
+        int pos = 0;
+        if ((int)cs.size() > cost_cap) {
+            rc = -2;
+            return 0;
+        }
+        for (size_t i = 0; i < cs.size(); ++i) {
+            const ScheduleCandidate& c = cs[i];
+            int need = 2 + n + 2 * (int)c.pairs.size();
+            if (pos + need > cap) {
+                rc = -2;
+                return 0;
+            }
+            out[pos++] = c.use_iep ? 1 : 0;
+            out[pos++] = (int32_t)c.pairs.size();
+            for (int v : c.order) out[pos++] = v;
+            for (const auto& pr : c.pairs) {
+                out[pos++] = pr.first;
+                out[pos++] = pr.second;
+            }
+            costs[i] = c.analytic_cost;
+        }
+        rc = (int)cs.size();
+        return 0;
+    });
+    return rc;
+}
+
+GS_API void* gs_schedule_from_candidate(int n, const char* adj, const int32_t* order, int npairs, const int32_t* pairs,
+                                        int iep) {
+    void* out = nullptr;
+    guarded([&] {
+        Pattern p(n, adj);
+        ScheduleCandidate c;
+        c.order.assign(order, order + n);
+        for (int i = 0; i < npairs; ++i) c.pairs.push_back(std::make_pair(pairs[2 * i], pairs[2 * i + 1]));
+        c.use_iep = iep != 0;
+        out = new GsSchedule{new Schedule_IEP(p, c), FlatPlan(), -1};
         return 0;
     });
     return out;
@@ -246,4 +306,128 @@ GS_API int gs_motifs(int k, char* out, int cap) {
     for (const auto& p : v)
         for (int i = 0; i < k * k; ++i) out[pos++] = p.get_adj_mat_ptr()[i] ? '1' : '0';
     return (int)v.size();
+}
+
+// ---- autotuner ---------------------------------------------------------------
+
+// Fills out[0..5] = waves_per_cu, chunk_edges, lane_tail, sample_ms*1000, from_cache, tried.
+GS_API int gs_autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_t ng, const gm_plan* const* plans,
+                             int32_t np, const char* key, const char* cache_path, int64_t* out) {
+    return guarded([&] {
+        TuneChoice c = autotune_batch(g, edge_offsets, ng, plans, np, key, cache_path);
+        out[0] = c.waves_per_cu;
+        out[1] = c.chunk_edges;
+        out[2] = c.lane_tail;
+        out[3] = (int64_t)(c.sample_ms * 1000.0);
+        out[4] = c.from_cache;
+        out[5] = c.tried;
+        return 0;
+    });
+}
+
+GS_API int gs_degree_class(const int64_t* row_ptr, int32_t n, char* out, int cap) {
+    std::string s = degree_class(row_ptr, n);
+    if ((int)s.size() + 1 > cap) return -1;
+    std::memcpy(out, s.c_str(), s.size() + 1);
+    return 0;
+}
+
+// ---- learned schedule cost model -------------------------------------------
+
+// Features of every candidate (the order of gs_schedule_candidates with
+// v_cnt = n, e_cnt = row_ptr[n]); rows of kFeatures floats.  Returns the
+// number of candidates, -1 on error, -2 if `cap_rows` is too small.
+GS_API int gs_encode_candidates(int n, const char* adj, int iep, const int64_t* row_ptr, int32_t nv, int64_t tri_cnt,
+                                float* out, int cap_rows) {
+    int rc = -1;
+    guarded([&] {
+        Pattern p(n, adj);
+        GraphStats st = graph_stats(row_ptr, nv, tri_cnt);
+        std::vector<ScheduleCandidate> cs = Schedule_IEP::enumerate_candidates(p, iep != 0, nv, st.nnz, tri_cnt);
+        if ((int)cs.size() > cap_rows) {
+            rc = -2;
+            return 0;
+        }
+        std::vector<float> x = encode_candidates(p, cs, st);
+        std::memcpy(out, x.data(), sizeof(float) * x.size());
+        rc = (int)cs.size();
+        return 0;
+    });
+    return rc;
+}
+
+GS_API int gs_feature_count() { return kFeatures; }
+
+GS_API void* gs_costmodel_new(int hidden, uint64_t seed) {
+    void* out = nullptr;
+    guarded([&] {
+        out = new CostModel(hidden, seed);
+        return 0;
+    });
+    return out;
+}
+
+GS_API void* gs_costmodel_load(const char* path) {
+    void* out = nullptr;
+    guarded([&] {
+        std::unique_ptr<CostModel> m = CostModel::load(path);
+        if (!m) throw std::runtime_error(std::string("no usable cost model at ") + path);
+        out = m.release();
+        return 0;
+    });
+    return out;
+}
+
+GS_API void gs_costmodel_free(void* h) { delete (CostModel*)h; }
+
+GS_API int gs_costmodel_save(void* h, const char* path) {
+    return guarded([&] {
+        ((CostModel*)h)->save(path);
+        return 0;
+    });
+}
+
+GS_API int gs_costmodel_train(void* h, const float* x, const float* t, const int32_t* goff, int32_t groups,
+                              int32_t steps, float lr, float wd, int fit_norm, float* loss) {
+    return guarded([&] {
+        CostModel* m = (CostModel*)h;
+        const int64_t rows = goff[groups];
+        std::vector<float> xv(x, x + rows * kFeatures), tv(t, t + rows);
+        std::vector<int32_t> go(goff, goff + groups + 1);
+        if (fit_norm) m->fit_normalisation(xv, rows);
+        *loss = m->train(xv, tv, go, steps, lr, wd);
+        return 0;
+    });
+}
+
+GS_API int gs_costmodel_predict(void* h, const float* x, int32_t rows, float* y) {
+    return guarded([&] {
+        std::vector<float> xv(x, x + (int64_t)rows * kFeatures);
+        std::vector<float> out = ((CostModel*)h)->predict(xv, rows);
+        std::memcpy(y, out.data(), sizeof(float) * out.size());
+        return 0;
+    });
+}
+
+// Schedule chosen by `model` (null: the analytic pick) among the candidates;
+// info = {candidates, chosen, analytic pick, learned}, *select_ms = time spent.
+GS_API void* gs_schedule_select(int n, const char* adj, int iep, const int64_t* row_ptr, int32_t nv, int64_t tri_cnt,
+                                void* model, int32_t* info, double* select_ms) {
+    void* out = nullptr;
+    guarded([&] {
+        Pattern p(n, adj);
+        GraphStats st = graph_stats(row_ptr, nv, tri_cnt);
+        ScheduleChoice ch;
+        Schedule_IEP* s = select_schedule(p, st, (CostModel*)model, iep != 0, &ch);
+        out = new GsSchedule{s, FlatPlan(), -1};
+        if (info) {
+            info[0] = ch.candidates;
+            info[1] = ch.chosen;
+            info[2] = ch.analytic;
+            info[3] = ch.learned ? 1 : 0;
+        }
+        if (select_ms) *select_ms = ch.select_ms;
+        return 0;
+    });
+    return out;
 }

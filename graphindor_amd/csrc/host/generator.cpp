@@ -172,3 +172,101 @@ int32_t second_largest_degree(const Graph& g) {
     }
     return (int32_t)d2;
 }
+
+namespace {
+
+// One graph of the batch, local ids [0, n).
+void batch_member(uint64_t seed, double log2_min, double log2_max, double deg_min, double deg_max,
+                  std::vector<std::pair<int32_t, int32_t> >& pairs, int32_t& n_out) {
+    uint64_t st = stream_seed(seed, 0x5eed5eedULL);
+    const bool rmat = (splitmix_next(st) & 1) != 0;
+    const double lg = log2_min + unit(st) * (log2_max - log2_min);
+    const int32_t n = std::max<int32_t>(2, (int32_t)std::llround(std::pow(2.0, lg)));
+    const double d = deg_min + unit(st) * (deg_max - deg_min);
+    const int64_t m = (int64_t)std::llround(d * n / 2.0);
+    n_out = n;
+    pairs.clear();
+    if (rmat) {
+        int scale = 1;
+        while ((1LL << scale) < n) ++scale;
+        const double a = 0.57, ab = 0.76, abc = 0.95;
+        for (int64_t i = 0; i < m; ++i) {
+            int32_t u, v;
+            uint64_t es = stream_seed(seed, (uint64_t)i + 1);
+            do {
+                u = 0;
+                v = 0;
+                for (int l = 0; l < scale; ++l) {
+                    double r = unit(es);
+                    int bu = 0, bv = 0;
+                    if (r < a) {
+                    } else if (r < ab) {
+                        bv = 1;
+                    } else if (r < abc) {
+                        bu = 1;
+                    } else {
+                        bu = bv = 1;
+                    }
+                    u = (u << 1) | bu;
+                    v = (v << 1) | bv;
+                }
+            } while (u >= n || v >= n);
+            pairs.push_back(std::make_pair(u, v));
+        }
+    } else {
+        const double p = std::min(1.0, d / (double)(n - 1));
+        const double lq = std::log1p(-p);
+        for (int32_t u = 0; u < n && p > 0; ++u) {
+            uint64_t rs = stream_seed(seed, (uint64_t)u + 1);
+            int64_t v = u;
+            while (true) {
+                double r = unit(rs);
+                v += 1 + (p >= 1.0 ? 0 : (int64_t)std::floor(std::log1p(-r) / lq));
+                if (v >= n) break;
+                pairs.push_back(std::make_pair(u, (int32_t)v));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+Graph* generate_batch(int count, uint64_t seed_base, double log2_min, double log2_max, double deg_min,
+                      double deg_max, int64_t* vertex_off, int64_t* edge_off) {
+    std::vector<Graph*> parts(count, nullptr);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < count; ++i) {
+        std::vector<std::pair<int32_t, int32_t> > pairs;
+        int32_t n = 0;
+        batch_member(seed_base + (uint64_t)i, log2_min, log2_max, deg_min, deg_max, pairs, n);
+        parts[i] = build_csr(n, pairs);
+    }
+    int64_t nv = 0, ne = 0;
+    for (int i = 0; i < count; ++i) {
+        vertex_off[i] = nv;
+        edge_off[i] = ne;
+        nv += parts[i]->v_cnt;
+        ne += parts[i]->e_cnt;
+    }
+    vertex_off[count] = nv;
+    edge_off[count] = ne;
+    Graph* g = new Graph();
+    g->v_cnt = (v_index_t)nv;
+    g->e_cnt = ne;
+    g->vertex = new e_index_t[(size_t)nv + 1];
+    g->edge = new v_index_t[ne > 0 ? (size_t)ne : 1];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < count; ++i) {
+        const Graph* p = parts[i];
+        for (v_index_t v = 0; v < p->v_cnt; ++v) g->vertex[vertex_off[i] + v] = edge_off[i] + p->vertex[v];
+        for (e_index_t e = 0; e < p->e_cnt; ++e) g->edge[edge_off[i] + e] = (v_index_t)(vertex_off[i] + p->edge[e]);
+    }
+    g->vertex[nv] = ne;
+    long long tri = 0;
+    for (int i = 0; i < count; ++i) {
+        tri += count_triangles(*parts[i]);
+        delete parts[i];
+    }
+    g->tri_cnt = tri;
+    return g;
+}

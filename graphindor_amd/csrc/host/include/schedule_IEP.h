@@ -39,6 +39,17 @@
 
 enum class RestrictValidation { Exact = 0, Reference = 1 };
 
+// One candidate schedule of a pattern, as the analytic selector sees it: a
+// vertex order (new vertex k is pattern vertex order[k]), a restriction set
+// over the reordered vertices (emb[second] < emb[first]), whether the IEP
+// tail is used, and the analytic cost of the pair.
+struct ScheduleCandidate {
+    std::vector<int> order;
+    std::vector<std::pair<int, int> > pairs;
+    bool use_iep = true;
+    double analytic_cost = 0;
+};
+
 class Schedule_IEP {
 public:
     // performance_modeling_type: 0 = keep the given order, 1 = analytic model
@@ -50,6 +61,18 @@ public:
     // Order as given, no restrictions, no IEP, vertex-induced
     // (src/schedule_IEP.cpp:249-291).
     Schedule_IEP(const int* _adj_mat, int _size);
+    // Exactly the given candidate (order, restrictions, IEP on/off).  The
+    // restriction set is validated exactly; an invalid one throws
+    // std::invalid_argument.  Lets a selector other than the analytic model
+    // choose among the candidates of enumerate_candidates().
+    Schedule_IEP(const Pattern& pattern, const ScheduleCandidate& candidate);
+    // Every (order, restriction set) the analytic selector of the first
+    // constructor weighs with performance_modeling_type = 1 and
+    // restricts_type = 1 (deepest-IEP connected orders, each exactly valid
+    // restriction set; the orders without IEP when none survives), with its
+    // analytic cost, in the order the selector visits them.  Exact mode.
+    static std::vector<ScheduleCandidate> enumerate_candidates(const Pattern& pattern, bool use_in_exclusion_optimize,
+                                                               int v_cnt, int64_t e_cnt, long long tri_cnt);
     Schedule_IEP(const Schedule_IEP&) = delete;
     Schedule_IEP& operator=(const Schedule_IEP&) = delete;
     ~Schedule_IEP();

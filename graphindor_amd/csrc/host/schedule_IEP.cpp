@@ -343,6 +343,64 @@ Schedule_IEP::Schedule_IEP(const int* _adj_mat, int _size) {
     set_in_exclusion_optimize_redundancy();
 }
 
+Schedule_IEP::Schedule_IEP(const Pattern& pattern, const ScheduleCandidate& c) {
+    is_vertex_induced = false;
+    size = pattern.get_size();
+    allocate_tables();
+    if ((int)c.order.size() != size) throw std::invalid_argument("Schedule_IEP: candidate order has the wrong length");
+    std::vector<int> seen(size, 0);
+    for (int v : c.order) {
+        if (v < 0 || v >= size || seen[v]++) throw std::invalid_argument("Schedule_IEP: candidate order is not a permutation");
+    }
+    for (const auto& pr : c.pairs)
+        if (pr.first < 0 || pr.first >= size || pr.second < 0 || pr.second >= size || pr.first == pr.second)
+            throw std::invalid_argument("Schedule_IEP: candidate restriction out of range");
+    copy_adj_mat_from(c.order, pattern.get_adj_mat_ptr());
+    if (!check_connectivity()) throw std::invalid_argument("Schedule_IEP: candidate order is not connected");
+    setup_optimization_info(c.use_iep);
+    if (!restrictions_exactly_valid(adj_mat, size, c.pairs) ||
+        (in_exclusion_optimize_num >= 2 && iep_head_multiplicity(adj_mat, size, c.pairs, in_exclusion_optimize_num) <= 0))
+        throw std::invalid_argument("Schedule_IEP: candidate restriction set does not break every automorphism once");
+    add_restrict(c.pairs);
+    set_in_exclusion_optimize_redundancy();
+}
+
+std::vector<ScheduleCandidate> Schedule_IEP::enumerate_candidates(const Pattern& pattern, bool use_iep, int v_cnt,
+                                                                  int64_t e_cnt, long long tri_cnt) {
+    if (tri_cnt < 0) throw std::logic_error("enumerate_candidates: the triangle count is required");
+    const int n = pattern.get_size();
+    const int* src = pattern.get_adj_mat_ptr();
+    Schedule_IEP s(src, n);  // scratch whose tables each order rebuilds
+    std::vector<std::vector<int> > orders = all_permutations(n);
+    s.remove_invalid_permutation(orders);
+    int deepest = 0;
+    for (const auto& vec : orders) deepest = std::max(deepest, s.get_vec_optimize_num(vec));
+    std::vector<std::vector<int> > kept;
+    for (const auto& vec : orders)
+        if (s.get_vec_optimize_num(vec) == deepest) kept.push_back(vec);
+    std::vector<ScheduleCandidate> out;
+    for (int pass = 0; pass < 2 && out.empty(); ++pass) {
+        const bool iep = pass == 0 && use_iep;
+        if (pass == 1 && !use_iep) break;
+        for (const auto& vec : kept) {
+            s.copy_adj_mat_from(vec, src);
+            s.setup_optimization_info(iep);
+            std::vector<std::vector<std::pair<int, int> > > sets;
+            s.restricts_generate(s.adj_mat, sets);
+            for (const auto& pairs : sets) {
+                ScheduleCandidate c;
+                c.order = vec;
+                c.pairs = pairs;
+                c.use_iep = iep;
+                c.analytic_cost = s.new_estimate_schedule_restrict(pairs, v_cnt, e_cnt, tri_cnt);
+                out.push_back(c);
+            }
+        }
+    }
+    if (out.empty()) throw std::runtime_error("enumerate_candidates: no valid restriction set for any vertex order");
+    return out;
+}
+
 Schedule_IEP::~Schedule_IEP() {
     delete[] adj_mat;
     delete[] father_prefix_id;

@@ -53,7 +53,7 @@ class GmPlanDesc(ctypes.Structure):
 class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
-                ("lane_tail", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("lane_tail", ctypes.c_int32), ("root_stride", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class GmStats(ctypes.Structure):
@@ -94,6 +94,29 @@ def gm():
         lib.gm_count_multi.argtypes = [VP, ctypes.POINTER(VP), ctypes.c_int32, ctypes.POINTER(GmTuning), ctypes.c_int32,
                                        VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(GmStats)]
         lib.gm_count_multi.restype = ctypes.c_int
+        lib.gm_count_batch.argtypes = [VP, VP, ctypes.c_int32, ctypes.POINTER(VP), ctypes.c_int32,
+                                       ctypes.POINTER(GmTuning), ctypes.c_int32, VP, ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.POINTER(GmStats)]
+        lib.gm_count_batch.restype = ctypes.c_int
+        lib.gm_batch_create.argtypes = [VP, VP, ctypes.c_int32, ctypes.POINTER(VP), ctypes.c_int32,
+                                        ctypes.POINTER(GmTuning), ctypes.c_int32, ctypes.POINTER(VP)]
+        lib.gm_batch_create.restype = ctypes.c_int
+        lib.gm_batch_run.argtypes = [VP, VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(GmStats)]
+        lib.gm_batch_run.restype = ctypes.c_int
+        lib.gm_batch_destroy.argtypes = [VP]
+        lib.gm_batch_destroy.restype = None
+        lib.gm_mlp_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(VP)]
+        lib.gm_mlp_destroy.argtypes = [VP]
+        lib.gm_mlp_destroy.restype = None
+        lib.gm_mlp_param_count.argtypes = [VP]
+        lib.gm_mlp_param_count.restype = ctypes.c_int64
+        lib.gm_mlp_get_params.argtypes = [VP, VP]
+        lib.gm_mlp_set_params.argtypes = [VP, VP]
+        lib.gm_mlp_get_grads.argtypes = [VP, VP]
+        lib.gm_mlp_predict.argtypes = [VP, VP, ctypes.c_int32, VP]
+        lib.gm_mlp_train.argtypes = [VP, VP, VP, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                     ctypes.POINTER(ctypes.c_float)]
+        lib.gm_mlp_last_error.restype = ctypes.c_char_p
         _gm = lib
     return _gm
 
@@ -112,6 +135,8 @@ def gs():
             ("gs_graph_rmat", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                ctypes.c_uint64, ctypes.c_int], VP),
             ("gs_graph_er", [ctypes.c_int32, ctypes.c_double, ctypes.c_uint64], VP),
+            ("gs_graph_batch", [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, VP, VP], VP),
             ("gs_graph_free", [VP], None),
             ("gs_graph_info", [VP, ctypes.POINTER(ctypes.c_int64)], None),
             ("gs_graph_csr", [VP, ctypes.POINTER(VP), ctypes.POINTER(VP)], None),
@@ -137,6 +162,24 @@ def gs():
             ("gs_gpu_count", [VP, VP, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
             ("gs_motifs", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+            ("gs_autotune_batch", [VP, VP, ctypes.c_int32, ctypes.POINTER(VP), ctypes.c_int32, ctypes.c_char_p,
+                                   ctypes.c_char_p, VP], ctypes.c_int),
+            ("gs_degree_class", [VP, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+            ("gs_schedule_candidates", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int32, ctypes.c_int64,
+                                        ctypes.c_int64, VP, ctypes.c_int, VP, ctypes.c_int], ctypes.c_int),
+            ("gs_schedule_from_candidate", [ctypes.c_int, ctypes.c_char_p, VP, ctypes.c_int, VP, ctypes.c_int], VP),
+            ("gs_encode_candidates", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64,
+                                      VP, ctypes.c_int], ctypes.c_int),
+            ("gs_feature_count", [], ctypes.c_int),
+            ("gs_costmodel_new", [ctypes.c_int, ctypes.c_uint64], VP),
+            ("gs_costmodel_load", [ctypes.c_char_p], VP),
+            ("gs_costmodel_free", [VP], None),
+            ("gs_costmodel_save", [VP, ctypes.c_char_p], ctypes.c_int),
+            ("gs_costmodel_train", [VP, VP, VP, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+            ("gs_costmodel_predict", [VP, VP, ctypes.c_int32, VP], ctypes.c_int),
+            ("gs_schedule_select", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64, VP,
+                                    VP, ctypes.POINTER(ctypes.c_double)], VP),
         ]:
             f = getattr(lib, name)
             f.argtypes = args
@@ -173,6 +216,15 @@ class Graph:
     @classmethod
     def erdos_renyi(cls, n, p, seed):
         return cls(gs().gs_graph_er(n, p, seed))
+
+    @classmethod
+    def batch(cls, count, seed_base, log2_min=10.0, log2_max=17.0, deg_min=4.0, deg_max=64.0):
+        """(union graph, vertex offsets, edge offsets) of a seeded batch."""
+        voff = np.zeros(count + 1, dtype=np.int64)
+        eoff = np.zeros(count + 1, dtype=np.int64)
+        h = gs().gs_graph_batch(count, seed_base, log2_min, log2_max, deg_min, deg_max, voff.ctypes.data,
+                                eoff.ctypes.data)
+        return cls(h), voff, eoff
 
     @classmethod
     def from_csr(cls, vertex, edge, tri_cnt=0):
@@ -241,6 +293,15 @@ class Schedule:
     def plain(cls, n, adj):
         return cls(gs().gs_schedule_new_plain(n, adj.encode()))
 
+    @classmethod
+    def from_candidate(cls, n, adj, cand):
+        """Exactly the given Candidate (Schedule_IEP(pattern, candidate))."""
+        order = (ctypes.c_int32 * n)(*cand.order)
+        flat = [x for p in cand.pairs for x in p]
+        pairs = (ctypes.c_int32 * (len(flat) + 1))(*flat)
+        return cls(gs().gs_schedule_from_candidate(n, adj.encode(), order, len(cand.pairs), pairs,
+                                                   1 if cand.use_iep else 0))
+
     def free(self):
         if self.h:
             gs().gs_schedule_free(self.h)
@@ -275,6 +336,111 @@ class Schedule:
 
     def print(self):
         gs().gs_schedule_print(self.h)
+
+
+class Candidate:
+    """One (order, restriction set, IEP) candidate with its analytic cost."""
+
+    __slots__ = ("order", "pairs", "use_iep", "analytic_cost")
+
+    def __init__(self, order, pairs, use_iep, analytic_cost):
+        self.order, self.pairs, self.use_iep, self.analytic_cost = tuple(order), tuple(pairs), use_iep, analytic_cost
+
+    def __repr__(self):
+        return f"Candidate(order={self.order}, pairs={self.pairs}, iep={self.use_iep}, cost={self.analytic_cost:.4g})"
+
+
+def schedule_candidates(n, adj, v_cnt, e_cnt, tri_cnt, iep=True):
+    """Every candidate the analytic selector weighs (Schedule_IEP::enumerate_candidates)."""
+    cap, ccap = 1 << 20, 1 << 15
+    buf = (ctypes.c_int32 * cap)()
+    costs = (ctypes.c_double * ccap)()
+    k = gs().gs_schedule_candidates(n, adj.encode(), 1 if iep else 0, v_cnt, e_cnt, tri_cnt, buf, cap, costs, ccap)
+    if k < 0:
+        raise GmError(gs().gs_last_error().decode() if k == -1 else "too many candidates")
+    out, pos = [], 0
+    for i in range(k):
+        use_iep, npairs = bool(buf[pos]), buf[pos + 1]
+        order = list(buf[pos + 2:pos + 2 + n])
+        fl = list(buf[pos + 2 + n:pos + 2 + n + 2 * npairs])
+        pos += 2 + n + 2 * npairs
+        out.append(Candidate(order, [(fl[2 * j], fl[2 * j + 1]) for j in range(npairs)], use_iep, costs[i]))
+    return out
+
+
+def encode_candidates(n, adj, vertex, tri_cnt, iep=True):
+    """Feature rows (float32 [candidates, F]) in the order of schedule_candidates."""
+    v = np.ascontiguousarray(vertex, dtype=np.int64)
+    F = gs().gs_feature_count()
+    cap = 1 << 14
+    out = np.empty((cap, F), dtype=np.float32)
+    k = gs().gs_encode_candidates(n, adj.encode(), 1 if iep else 0, v.ctypes.data, len(v) - 1, tri_cnt,
+                                  out.ctypes.data, cap)
+    if k < 0:
+        raise GmError(gs().gs_last_error().decode() if k == -1 else "too many candidates")
+    return out[:k].copy()
+
+
+class CostModel:
+    """The learned schedule cost model (host CostModel over the gm_mlp kernels)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise GmError(gs().gs_last_error().decode() or "cost model creation failed")
+        self.h = handle
+
+    @classmethod
+    def new(cls, hidden=256, seed=6):
+        return cls(gs().gs_costmodel_new(hidden, seed))
+
+    @classmethod
+    def load(cls, path):
+        return cls(gs().gs_costmodel_load(str(path).encode()))
+
+    def save(self, path):
+        if gs().gs_costmodel_save(self.h, str(path).encode()) != 0:
+            raise GmError(gs().gs_last_error().decode())
+
+    def train(self, x, log_ms, group_off, steps, lr=1e-3, weight_decay=1e-4, fit_norm=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(log_ms, dtype=np.float32)
+        go = np.ascontiguousarray(group_off, dtype=np.int32)
+        loss = ctypes.c_float(0)
+        if gs().gs_costmodel_train(self.h, x.ctypes.data, t.ctypes.data, go.ctypes.data, len(go) - 1, steps, lr,
+                                   weight_decay, 1 if fit_norm else 0, ctypes.byref(loss)) != 0:
+            raise GmError(gs().gs_last_error().decode())
+        return loss.value
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        if gs().gs_costmodel_predict(self.h, x.ctypes.data, x.shape[0], y.ctypes.data) != 0:
+            raise GmError(gs().gs_last_error().decode())
+        return y
+
+    def free(self):
+        if self.h:
+            gs().gs_costmodel_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def select_schedule(n, adj, vertex, tri_cnt, model=None, iep=True):
+    """(Schedule, info) chosen by `model` among the analytic selector's
+    candidates, or the analytic pick when model is None."""
+    v = np.ascontiguousarray(vertex, dtype=np.int64)
+    info = (ctypes.c_int32 * 4)()
+    ms = ctypes.c_double(0)
+    h = gs().gs_schedule_select(n, adj.encode(), 1 if iep else 0, v.ctypes.data, len(v) - 1, tri_cnt,
+                                model.h if model is not None else None, info, ctypes.byref(ms))
+    s = Schedule(h)
+    return s, {"candidates": info[0], "chosen": info[1], "analytic": info[2], "learned": bool(info[3]),
+               "select_ms": ms.value}
 
 
 def set_restrict_validation(mode):
@@ -352,3 +518,154 @@ def count(dgraph, dplan, flags=0, waves_per_cu=0, bitmap_max_degree=0, lane_tail
     _gm_check(gm().gm_count(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw), ctypes.byref(st)),
               "gm_count")
     return raw.value, st
+
+
+def _tuning(waves_per_cu=0, bitmap_max_degree=0, lane_tail=0, chunk_edges=0, root_stride=0):
+    tun = GmTuning()
+    tun.root_stride = root_stride
+    tun.waves_per_cu = waves_per_cu
+    tun.bitmap_max_degree = bitmap_max_degree
+    tun.lane_tail = lane_tail
+    tun.chunk_edges = chunk_edges
+    return tun
+
+
+def count_multi(dgraph, dplans, flags=0, **tuning):
+    """Raw sums of several plans over one graph in one call (gm_count_multi)."""
+    arr = (VP * len(dplans))(*[p.h for p in dplans])
+    out = (ctypes.c_uint64 * len(dplans))()
+    st = GmStats()
+    tun = _tuning(**tuning)
+    _gm_check(gm().gm_count_multi(dgraph.h, arr, len(dplans), ctypes.byref(tun), flags, None, out, ctypes.byref(st)),
+              "gm_count_multi")
+    return list(out), st
+
+
+def count_batch(dgraph, edge_offsets, dplans, flags=0, **tuning):
+    """Raw sums [graph][plan] over a batch held as one CSR (gm_count_batch)."""
+    eoff = np.ascontiguousarray(edge_offsets, dtype=np.int64)
+    ng = len(eoff) - 1
+    arr = (VP * len(dplans))(*[p.h for p in dplans])
+    out = (ctypes.c_uint64 * (ng * len(dplans)))()
+    st = GmStats()
+    tun = _tuning(**tuning)
+    _gm_check(gm().gm_count_batch(dgraph.h, eoff.ctypes.data, ng, arr, len(dplans), ctypes.byref(tun), flags, None,
+                                  out, ctypes.byref(st)), "gm_count_batch")
+    vals = list(out)
+    return [vals[i * len(dplans):(i + 1) * len(dplans)] for i in range(ng)], st
+
+
+class Batch:
+    """A batch count captured into a hipGraph (gm_batch_create / gm_batch_run)."""
+
+    def __init__(self, dgraph, edge_offsets, dplans, flags=0, **tuning):
+        self.eoff = np.ascontiguousarray(edge_offsets, dtype=np.int64)
+        self.ng = len(self.eoff) - 1
+        self.np = len(dplans)
+        self._keep = (dgraph, dplans)
+        arr = (VP * self.np)(*[p.h for p in dplans])
+        tun = _tuning(**tuning)
+        h = VP()
+        _gm_check(gm().gm_batch_create(dgraph.h, self.eoff.ctypes.data, self.ng, arr, self.np, ctypes.byref(tun),
+                                       flags, ctypes.byref(h)), "gm_batch_create")
+        self.h = h
+
+    def run(self):
+        out = (ctypes.c_uint64 * (self.ng * self.np))()
+        st = GmStats()
+        _gm_check(gm().gm_batch_run(self.h, None, out, ctypes.byref(st)), "gm_batch_run")
+        vals = list(out)
+        return [vals[i * self.np:(i + 1) * self.np] for i in range(self.ng)], st
+
+    def free(self):
+        if self.h:
+            gm().gm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def degree_class(vertex):
+    v = np.ascontiguousarray(vertex, dtype=np.int64)
+    buf = ctypes.create_string_buffer(64)
+    gs().gs_degree_class(v.ctypes.data, len(v) - 1, buf, 64)
+    return buf.value.decode()
+
+
+def autotune_batch(dgraph, edge_offsets, dplans, key, cache_path=None):
+    """Fastest tuning on a sample of the batch (cached per key); returns a dict."""
+    eoff = np.ascontiguousarray(edge_offsets, dtype=np.int64)
+    arr = (VP * len(dplans))(*[p.h for p in dplans])
+    out = (ctypes.c_int64 * 6)()
+    rc = gs().gs_autotune_batch(dgraph.h, eoff.ctypes.data, len(eoff) - 1, arr, len(dplans), key.encode(),
+                                cache_path.encode() if cache_path else None, out)
+    if rc != 0:
+        raise GmError(gs().gs_last_error().decode())
+    return {"waves_per_cu": out[0], "chunk_edges": out[1], "lane_tail": out[2], "sample_ms": out[3] / 1000.0,
+            "from_cache": bool(out[4]), "tried": out[5]}
+
+
+class Mlp:
+    """The cost model's FP32 MLP on the GPU (gm_mlp_*): F inputs, 2 x H ReLU, 1 output."""
+
+    def __init__(self, in_dim, hidden=256, seed=6):
+        h = VP()
+        rc = gm().gm_mlp_create(in_dim, hidden, seed, ctypes.byref(h))
+        if rc != 0:
+            raise GmError(gm().gm_mlp_last_error().decode())
+        self.h, self.in_dim, self.hidden = h, in_dim, hidden
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise GmError(f"{what}: {gm().gm_mlp_last_error().decode()}")
+
+    @property
+    def nparams(self):
+        return gm().gm_mlp_param_count(self.h)
+
+    def get_params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        self._check(gm().gm_mlp_get_params(self.h, out.ctypes.data), "gm_mlp_get_params")
+        return out
+
+    def get_grads(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        self._check(gm().gm_mlp_get_grads(self.h, out.ctypes.data), "gm_mlp_get_grads")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        self._check(gm().gm_mlp_set_params(self.h, p.ctypes.data), "gm_mlp_set_params")
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert x.ndim == 2 and x.shape[1] == self.in_dim
+        y = np.empty(x.shape[0], dtype=np.float32)
+        self._check(gm().gm_mlp_predict(self.h, x.ctypes.data, x.shape[0], y.ctypes.data), "gm_mlp_predict")
+        return y
+
+    def train(self, x, t, group_off, steps, lr=1e-3, weight_decay=0.0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        go = np.ascontiguousarray(group_off, dtype=np.int32)
+        assert x.shape == (go[-1], self.in_dim) and t.shape == (go[-1],)
+        loss = ctypes.c_float(0)
+        self._check(gm().gm_mlp_train(self.h, x.ctypes.data, t.ctypes.data, go.ctypes.data, len(go) - 1, steps, lr,
+                                      weight_decay, ctypes.byref(loss)), "gm_mlp_train")
+        return loss.value
+
+    def free(self):
+        if self.h:
+            gm().gm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -96,7 +96,9 @@ typedef struct gm_tuning {
                                   local-bitmap engine (0 = default 1024, -1 = off) */
     int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted lane-parallel
                                   (0 = default on, -1 = off: every depth wave-wide) */
-    int32_t reserved[3];
+    int32_t root_stride;       /* GM_MODE_PATTERN: count only every root_stride-th root edge of the
+                                  range (a uniform sample for timing; 0 or 1 = every edge) */
+    int32_t reserved[2];
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */
@@ -140,6 +142,53 @@ int gm_count(const gm_graph* g, const gm_plan* p, const gm_tuning* tuning, int32
 int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np,
                    const gm_tuning* tuning, int32_t flags, void* stream, uint64_t* raw_sums,
                    gm_stats* stats);
+
+/* Count np plans over a batch of ng graphs held as one CSR with globally
+ * numbered vertices (graph i owns the root edges [graph_edge_offsets[i],
+ * graph_edge_offsets[i+1]); offsets[0] = 0, offsets[ng] = nnz), in one
+ * persistent launch per counting rule.  raw_sums[i * np + j] is graph i's raw
+ * sum for plan j.  Replaces running gpu_mc once per graph (gpu/gpu_mc.cu:62-117). */
+int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng,
+                   const gm_plan* const* plans, int32_t np, const gm_tuning* tuning, int32_t flags,
+                   void* stream, uint64_t* raw_sums, gm_stats* stats);
+
+/* The same batch count captured once into a hipGraph (counter resets by
+ * kernels, then the fused persistent launch) and replayed by gm_batch_run.
+ * Pattern-rule plans only.  The graph and plans must outlive the batch; the
+ * batch owns its scratch, so runs of different batches do not share state. */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng,
+                    const gm_plan* const* plans, int32_t np, const gm_tuning* tuning, int32_t flags,
+                    gm_batch** out);
+int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats);
+void gm_batch_destroy(gm_batch* b);
+
+/* ---- Learned schedule cost model ------------------------------------------
+ * No counterpart in the reference: the schedule compiler's analytic model
+ * (new_estimate_schedule_restrict, src/schedule_IEP.cpp:307-382) ranks the
+ * candidate (order, restriction set) pairs by formula; this FP32 MLP ranks
+ * them from measured run times instead, with the analytic model as the
+ * fallback.  Network: F inputs, two ReLU layers of H units, one output;
+ * parameters laid out as W1[F*H] b1[H] W2[H*H] b2[H] W3[H] b3[1] (row-major,
+ * W1[f*H + h]).  Training minimises the group-centred squared error
+ * sum_g sum_{i in g} ((y_i - t_i) - mean_{j in g}(y_j - t_j))^2 / rows with
+ * full-batch Adam (beta 0.9/0.999, eps 1e-8, decoupled weight decay), one
+ * step captured in a hipGraph and replayed.  Rows of a group are contiguous:
+ * group g is rows [group_off[g], group_off[g+1]).  All calls synchronise. */
+typedef struct gm_mlp gm_mlp;
+int gm_mlp_create(int32_t in_dim, int32_t hidden, uint64_t seed, gm_mlp** out);
+void gm_mlp_destroy(gm_mlp* m);
+int64_t gm_mlp_param_count(const gm_mlp* m);
+int gm_mlp_get_params(const gm_mlp* m, float* out);
+/* Gradients of the last training step's loss, same layout as the parameters. */
+int gm_mlp_get_grads(const gm_mlp* m, float* out);
+/* Loads parameters and resets the optimiser state. */
+int gm_mlp_set_params(gm_mlp* m, const float* in);
+int gm_mlp_predict(gm_mlp* m, const float* x, int32_t rows, float* y);
+/* x[rows*F], t[rows]; final_loss (may be null) = loss before the last step. */
+int gm_mlp_train(gm_mlp* m, const float* x, const float* t, const int32_t* group_off, int32_t groups, int32_t steps,
+                 float lr, float weight_decay, float* final_loss);
+const char* gm_mlp_last_error(void);
 
 /* Number of visible HIP devices (0 without a GPU); never fails. */
 int gm_device_count(void);

@@ -289,3 +289,116 @@ def test_lane_tail_matches_wave_path(name):
         off, _ = N.count(dg, dp, lane_tail=-1)
         assert on == off
         gpu_vs_oracle(v, e, s)
+
+
+def test_count_multi_matches_single_plans():
+    """All six 4-motifs in one gm_count_multi call equal six gm_count calls."""
+    g = N.Graph.rmat(12, 16, 9)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in MOTIFS4]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    multi, st = N.count_multi(dg, plans, flags=N.GM_COUNT_ALG_BYTES)
+    single = [N.count(dg, p)[0] for p in plans]
+    assert multi == single
+    csr = O.CSR(v, e)
+    assert multi == [O.count_edge_rooted(csr, s.plan_desc())[0] for s in scheds]
+    # A clique plan mixed in goes through its own launch.
+    clique = N.Schedule.plain(4, clique_adj(4))
+    clique.redundancy = 1
+    d = N.Graph.from_csr(v, e).orient_by_degree()
+    dv, de = d.csr()
+    dgo = N.DeviceGraph(dv.copy(), de.copy())
+    mixed, _ = N.count_multi(dgo, [N.DevicePlan(clique.plan_desc(N.GM_MODE_CLIQUE))])
+    assert mixed[0] == multi[0] // scheds[0].redundancy  # K4 count, either rule
+
+
+@pytest.mark.parametrize("chunk_edges", [0, 64, 512])
+def test_count_batch_per_graph_sums(chunk_edges):
+    """A batch of seeded graphs as one CSR: per-graph, per-motif sums equal
+    counting each graph on its own (and the oracle)."""
+    g, voff, eoff = N.Graph.batch(12, 5, 8.0, 11.0, 4.0, 24.0)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in MOTIFS4]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    sums, st = N.count_batch(dg, eoff, plans, flags=N.GM_COUNT_ALG_BYTES, chunk_edges=chunk_edges)
+    assert len(sums) == 12
+    for i in range(12):
+        lv = v[voff[i]:voff[i + 1] + 1] - v[voff[i]]
+        le = e[eoff[i]:eoff[i + 1]] - voff[i]
+        csr = O.CSR(lv, le)
+        want = [O.count_edge_rooted(csr, s.plan_desc())[0] for s in scheds]
+        assert sums[i] == want, i
+    # the union graph's per-plan totals equal the per-graph sums added up
+    whole, _ = N.count_multi(dg, plans)
+    assert whole == [sum(sums[i][j] for i in range(12)) for j in range(6)]
+
+
+def test_graph_captured_batch_replays():
+    """The hipGraph batch: several replays, with eager counts and syncs in
+    between, each equal to the eager gm_count_batch result."""
+    g, voff, eoff = N.Graph.batch(10, 40, 8.0, 11.0, 4.0, 24.0)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in MOTIFS4]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    eager, _ = N.count_batch(dg, eoff, plans)
+    b = N.Batch(dg, eoff, plans)
+    for _ in range(4):
+        got, st = b.run()
+        assert got == eager
+        again, _ = N.count_batch(dg, eoff, plans)  # eager work between replays
+        assert again == eager
+    b.free()
+    # a root-edge sub-range (the autotuner's sample): graphs 2..5 only
+    sub, _ = N.count_batch(dg, eoff[2:7], plans)
+    assert sub == eager[2:6]
+
+
+def test_autotuner_choice_keeps_counts(tmp_path):
+    """The autotuner times its candidates on a sample of the batch, caches the
+    choice per key, and the chosen tuning gives the default tuning's sums."""
+    g, voff, eoff = N.Graph.batch(24, 77, 8.0, 11.0, 4.0, 24.0)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in MOTIFS4]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    cache = str(tmp_path / "tune.txt")
+    key = "motifs4:" + N.degree_class(v)
+    first = N.autotune_batch(dg, eoff, plans, key, cache)
+    assert not first["from_cache"] and first["tried"] == 18
+    again = N.autotune_batch(dg, eoff, plans, key, cache)
+    assert again["from_cache"] and again["waves_per_cu"] == first["waves_per_cu"]
+    want, _ = N.count_batch(dg, eoff, plans)
+    got, _ = N.count_batch(dg, eoff, plans, waves_per_cu=first["waves_per_cu"], chunk_edges=first["chunk_edges"],
+                           lane_tail=first["lane_tail"])
+    assert got == want
+
+
+@pytest.mark.parametrize("stride", [2, 7, 64])
+def test_root_stride_phases_partition_the_count(stride):
+    """Root-edge sampling: the sums over the residues r of edges
+    r, r + S, r + 2S, ... add up to the full count exactly."""
+    g = N.Graph.rmat(11, 8, 21)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    s = N.Schedule.build(5, PATTERNS["house"][1], inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
+    dp = N.DevicePlan(s.plan_desc())
+    full, _ = N.count(dg, dp)
+    parts = []
+    for r in range(stride):
+        got, _ = N.count_batch(dg, [r, len(e)], [dp], root_stride=stride)
+        parts.append(got[0][0])
+    assert sum(parts) == full
+    assert all(x <= full for x in parts)
