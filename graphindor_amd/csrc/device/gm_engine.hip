@@ -92,7 +92,8 @@ struct KernelArgs {
     int32_t tail_off;             // 1: never use the lane-parallel pattern tail
     int32_t chunk_mult;           // root edges per task = 64 * chunk_mult
     int64_t root_begin, root_end; // root edges counted: [gedge[0], gedge[ng])
-    int64_t root_stride;          // counted root edges: root_begin + i * root_stride < root_end
+    int64_t root_stride;          // counted root edges: root_begin + i * root_stride < root_end (root_begin is
+                                  // the range's first multiple of root_stride)
     int64_t nroots;               // number of counted root edges
     uint32_t* scratch;
     int64_t wave_stride;  // uint32 per wave
@@ -1121,8 +1122,9 @@ struct DeviceCache {
     size_t gedge_bytes = 0;
     void* sums = nullptr;
     size_t sums_bytes = 0;
-    Counters* ctr = nullptr;
+    Counters* ctr = nullptr;  // [0] call totals, [1] one clique launch, [2] one bitmap launch, [3] bitmap totals
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evpool;  // start/stop pairs around individual launches
 };
 
 DeviceCache& cache_for(int dev) {
@@ -1478,11 +1480,19 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     HIP_TRY(grow(&dc.plan_ptrs, &dc.plan_ptrs_bytes, sizeof(DevPlan*) * (size_t)np));
     HIP_TRY(grow(&dc.gedge, &dc.gedge_bytes, sizeof(int64_t) * (size_t)(ng + 1)));
     HIP_TRY(grow(&dc.sums, &dc.sums_bytes, sizeof(unsigned long long) * (size_t)ng * (size_t)np));
-    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * 2));
+    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * 4));
     if (!dc.ev0) {
         HIP_TRY(hipEventCreate(&dc.ev0));
         HIP_TRY(hipEventCreate(&dc.ev1));
     }
+    // Event pairs: the fused pattern launch, then a list and a bitmap launch per clique plan.
+    const size_t nev = 2 * (size_t)(1 + 2 * np_clq);
+    while (dc.evpool.size() < nev) {
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreate(&ev));
+        dc.evpool.push_back(ev);
+    }
+    std::vector<int> ev_list, ev_bitmap;  // indices of start events used
     std::vector<const DevPlan*> hptrs(np);
     for (int k = 0; k < np; ++k) hptrs[k] = plans[order[k]]->d_plan;
     HIP_TRY(hipMemcpyAsync(dc.plan_ptrs, hptrs.data(), sizeof(DevPlan*) * np, hipMemcpyHostToDevice, s));
@@ -1493,7 +1503,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     unsigned long long* d_sums = (unsigned long long*)dc.sums;
     const DevPlan* const* d_plans = (const DevPlan* const*)dc.plan_ptrs;
     const int64_t ncells = (int64_t)ng * np;
-    const int64_t nctr_words = (int64_t)(sizeof(Counters) * 2 / sizeof(unsigned long long));
+    const int64_t nctr_words = (int64_t)(sizeof(Counters) * 4 / sizeof(unsigned long long));
 
     HIP_TRY(hipEventRecord(dc.ev0, s));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, s, d_sums, ncells);
@@ -1515,10 +1525,12 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     a.stats = stats_on;
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.chunk_mult = chunk_mult;
-    a.root_begin = hgedge[0];
+    // Sampling is absolute: the counted roots are the edges e of the range
+    // with e % root_stride == 0, so splitting a range never changes the sample.
+    a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
     a.root_stride = root_stride;
-    a.nroots = (a.root_end - a.root_begin + root_stride - 1) / root_stride;
+    a.nroots = a.root_begin < a.root_end ? (a.root_end - a.root_begin + root_stride - 1) / root_stride : 0;
     const int64_t chunks = (a.nroots + (int64_t)kWave * chunk_mult - 1) / ((int64_t)kWave * chunk_mult);
     if (np_pat > 0) {
         a.plans = d_plans;
@@ -1526,13 +1538,19 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         a.sums = d_sums;
         a.nchunks = chunks * np_pat;
         a.root_min_outdeg = 0;
+        HIP_TRY(hipEventRecord(dc.evpool[0], s));
         hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, s, a);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(dc.evpool[1], s));
+        ev_list.push_back(0);
     }
     // Clique plans: one at a time; on a single graph the local-bitmap engine
     // takes the roots of out-degree <= bm_max first.
     Counters* ctr2 = dc.ctr + 1;
+    Counters* ctr_bm = dc.ctr + 2;
+    Counters* ctr_bm_total = dc.ctr + 3;
     for (int c = 0; c < np_clq; ++c) {
+        const int evl = 2 + 4 * c, evb = evl + 2;
         const gm_plan* p = plans[order[np_pat + c]];
         unsigned long long* cell = d_sums + (int64_t)ng * np_pat + c;  // stride 1 per graph below
         const bool use_bitmap = ng == 1 && !graph_edge_offsets && p->host.clique_chain && bm_max > 0;
@@ -1552,9 +1570,16 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             b.rows_scratch = (unsigned long long*)dc.bm_rows;
             b.nchunks = ((int64_t)g->n + kBmChunk - 1) / kBmChunk;
             b.out = cell;
-            b.ctr = ctr2;
+            b.ctr = ctr_bm;
             b.stats = stats_on;
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_bm,
+                               (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
+            HIP_TRY(hipEventRecord(dc.evpool[evb], s));
             HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
+            HIP_TRY(hipEventRecord(dc.evpool[evb + 1], s));
+            ev_bitmap.push_back(evb);
+            hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr_bm);
+            hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm_total, ctr_bm);
             a.root_min_outdeg = bm_max;
         }
         // Per-graph cells of this plan are ng apart: give the kernel the
@@ -1567,19 +1592,34 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             // cells for graph g at cell[g * np_clq]: use a private region instead
             a.sums = d_sums + (int64_t)ng * np_pat + (int64_t)c * ng;
         }
+        HIP_TRY(hipEventRecord(dc.evpool[evl], s));
         hipLaunchKernelGGL(enumerate_kernel<GM_MODE_CLIQUE>, dim3(blocks), dim3(kBlock), 0, s, a);
         HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(dc.evpool[evl + 1], s));
+        ev_list.push_back(evl);
         hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr2);
         a.ctr = dc.ctr;
     }
     HIP_TRY(hipEventRecord(dc.ev1, s));
     std::vector<unsigned long long> hs(ncells);
-    Counters c;
+    Counters c, cb;
     HIP_TRY(hipMemcpyAsync(hs.data(), d_sums, sizeof(unsigned long long) * ncells, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&c, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&cb, ctr_bm_total, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
+    double list_ms = 0, bitmap_ms = 0;
+    for (int i : ev_list) {
+        float x = 0;
+        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+        list_ms += x;
+    }
+    for (int i : ev_bitmap) {
+        float x = 0;
+        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+        bitmap_ms += x;
+    }
     if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
     // Scatter back to raw_sums[graph * np + plan] in the caller's plan order.
     for (int gi = 0; gi < ng; ++gi) {
@@ -1594,6 +1634,9 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         stats->root_tasks = c.tasks;
         stats->grid_waves = (int32_t)nwaves;
         stats->flags_used = flags;
+        stats->list_ms = list_ms;
+        stats->bitmap_ms = bitmap_ms;
+        stats->bitmap_alg_bytes = cb.bytes;
     }
     return GM_OK;
 }
@@ -1699,7 +1742,8 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.sums = (unsigned long long*)b->d_sums;
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.chunk_mult = (t && t->chunk_edges > 0) ? std::max(1, t->chunk_edges / kWave) : 4;
-    a.root_begin = hgedge[0];
+    const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
+    a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
     a.scratch = (uint32_t*)b->d_scratch;
     a.wave_stride = stride;
@@ -1707,8 +1751,8 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.cap_vi = cap_vi;
     a.ctr = (Counters*)b->d_ctr;
     a.stats = b->stats;
-    a.root_stride = 1;
-    a.nroots = a.root_end - a.root_begin;
+    a.root_stride = root_stride;
+    a.nroots = a.root_begin < a.root_end ? (a.root_end - a.root_begin + root_stride - 1) / root_stride : 0;
     a.nchunks = ((a.nroots + (int64_t)kWave * a.chunk_mult - 1) / ((int64_t)kWave * a.chunk_mult)) * np;
     // Capture: zero the cells and counters with kernels (no memset nodes),
     // then the fused launch.
@@ -1754,6 +1798,9 @@ int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats)
         stats->root_tasks = c.tasks;
         stats->grid_waves = (int32_t)b->nwaves;
         stats->flags_used = b->stats;
+        stats->list_ms = ms;
+        stats->bitmap_ms = 0;
+        stats->bitmap_alg_bytes = 0;
     }
     return GM_OK;
 }

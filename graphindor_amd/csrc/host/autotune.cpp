@@ -1,6 +1,7 @@
 // Batched-count autotuner (see autotune.h).
 #include "autotune.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -46,13 +47,12 @@ void cache_store(const char* path, const std::string& key, const TuneChoice& c) 
 }  // namespace
 
 TuneChoice autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_t ng, const gm_plan* const* plans,
-                          int32_t np, const std::string& key, const char* cache_path) {
+                          int32_t np, const std::string& key, const char* cache_path, int32_t root_stride) {
     TuneChoice best;
     if (cache_lookup(cache_path, key, best)) return best;
-    // Sample: the first graphs until ~5% of the root edges (at least 8 graphs).
-    const int64_t total = edge_offsets[ng] - edge_offsets[0];
-    int32_t k = 0;
-    while (k < ng && (k < 8 || edge_offsets[k] - edge_offsets[0] < total / 20)) ++k;
+    // Sample: every graph, every (16 * root_stride)-th root edge.
+    const int32_t sample_stride = std::max(1, root_stride) * 16;
+    const int32_t k = ng;
     std::vector<uint64_t> sums((size_t)k * np);
     const int waves[] = {8, 12, 16};
     const int chunks[] = {64, 256, 1024};
@@ -65,6 +65,7 @@ TuneChoice autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_
                 t.waves_per_cu = w;
                 t.chunk_edges = c;
                 t.lane_tail = tl;
+                t.root_stride = sample_stride;
                 gm_stats st{};
                 // one warm-up, then the timed run
                 if (gm_count_batch(g, edge_offsets, k, plans, np, &t, 0, nullptr, sums.data(), &st) != GM_OK ||

@@ -312,9 +312,9 @@ GS_API int gs_motifs(int k, char* out, int cap) {
 
 // Fills out[0..5] = waves_per_cu, chunk_edges, lane_tail, sample_ms*1000, from_cache, tried.
 GS_API int gs_autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_t ng, const gm_plan* const* plans,
-                             int32_t np, const char* key, const char* cache_path, int64_t* out) {
+                             int32_t np, const char* key, const char* cache_path, int32_t root_stride, int64_t* out) {
     return guarded([&] {
-        TuneChoice c = autotune_batch(g, edge_offsets, ng, plans, np, key, cache_path);
+        TuneChoice c = autotune_batch(g, edge_offsets, ng, plans, np, key, cache_path, root_stride);
         out[0] = c.waves_per_cu;
         out[1] = c.chunk_edges;
         out[2] = c.lane_tail;

@@ -4,10 +4,10 @@
 // lane-parallel last level on/off): how many wavefronts share the root
 // tasks, how large a task is, and whether the last candidate level runs
 // per lane.  For a (pattern set, degree class) the tuner times each
-// candidate on a sample of the batch -- its first graphs up to ~5% of the
-// root edges, which is an unbiased sample since batch members are drawn
-// independently -- keeps the fastest and remembers it in a cache file
-// (one line per key: key waves chunk tail ms).
+// candidate on a sample of the batch -- every graph, the root edges e with
+// e % (16 * root_stride) == 0, root_stride being the sampling of the runs
+// it tunes for (1 = full runs) -- keeps the fastest and remembers it in a
+// cache file (one line per key: key waves chunk tail ms).
 #pragma once
 #include <cstdint>
 #include <string>
@@ -28,4 +28,4 @@ struct TuneChoice {
 std::string degree_class(const int64_t* row_ptr, int32_t n);
 
 TuneChoice autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_t ng, const gm_plan* const* plans,
-                          int32_t np, const std::string& key, const char* cache_path);
+                          int32_t np, const std::string& key, const char* cache_path, int32_t root_stride = 1);

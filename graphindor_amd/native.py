@@ -58,7 +58,8 @@ class GmTuning(ctypes.Structure):
 
 class GmStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("alg_bytes", ctypes.c_uint64), ("root_tasks", ctypes.c_uint64),
-                ("grid_waves", ctypes.c_int32), ("flags_used", ctypes.c_int32)]
+                ("grid_waves", ctypes.c_int32), ("flags_used", ctypes.c_int32), ("list_ms", ctypes.c_double),
+                ("bitmap_ms", ctypes.c_double), ("bitmap_alg_bytes", ctypes.c_uint64)]
 
 
 _gm = None
@@ -163,7 +164,7 @@ def gs():
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
             ("gs_motifs", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
             ("gs_autotune_batch", [VP, VP, ctypes.c_int32, ctypes.POINTER(VP), ctypes.c_int32, ctypes.c_char_p,
-                                   ctypes.c_char_p, VP], ctypes.c_int),
+                                   ctypes.c_char_p, ctypes.c_int32, VP], ctypes.c_int),
             ("gs_degree_class", [VP, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
             ("gs_schedule_candidates", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int32, ctypes.c_int64,
                                         ctypes.c_int64, VP, ctypes.c_int, VP, ctypes.c_int], ctypes.c_int),
@@ -596,13 +597,13 @@ def degree_class(vertex):
     return buf.value.decode()
 
 
-def autotune_batch(dgraph, edge_offsets, dplans, key, cache_path=None):
+def autotune_batch(dgraph, edge_offsets, dplans, key, cache_path=None, root_stride=1):
     """Fastest tuning on a sample of the batch (cached per key); returns a dict."""
     eoff = np.ascontiguousarray(edge_offsets, dtype=np.int64)
     arr = (VP * len(dplans))(*[p.h for p in dplans])
     out = (ctypes.c_int64 * 6)()
     rc = gs().gs_autotune_batch(dgraph.h, eoff.ctypes.data, len(eoff) - 1, arr, len(dplans), key.encode(),
-                                cache_path.encode() if cache_path else None, out)
+                                cache_path.encode() if cache_path else None, root_stride, out)
     if rc != 0:
         raise GmError(gs().gs_last_error().decode())
     return {"waves_per_cu": out[0], "chunk_edges": out[1], "lane_tail": out[2], "sample_ms": out[3] / 1000.0,

@@ -96,18 +96,22 @@ typedef struct gm_tuning {
                                   local-bitmap engine (0 = default 1024, -1 = off) */
     int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted lane-parallel
                                   (0 = default on, -1 = off: every depth wave-wide) */
-    int32_t root_stride;       /* GM_MODE_PATTERN: count only every root_stride-th root edge of the
-                                  range (a uniform sample for timing; 0 or 1 = every edge) */
+    int32_t root_stride;       /* GM_MODE_PATTERN: count only the root edges e with e % root_stride == 0
+                                  (a uniform sample for timing, identical however the range is split;
+                                  0 or 1 = every edge) */
     int32_t reserved[2];
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */
 typedef struct gm_stats {
-    double kernel_ms;       /* HIP-event time of the counting kernel(s) on the stream */
-    uint64_t alg_bytes;     /* algorithmic bytes (only when GM_COUNT_ALG_BYTES is set) */
-    uint64_t root_tasks;    /* root edges the grid claimed */
-    int32_t grid_waves;     /* persistent waves launched */
+    double kernel_ms;         /* HIP-event time of the whole counting region on the stream */
+    uint64_t alg_bytes;       /* algorithmic bytes (only when GM_COUNT_ALG_BYTES is set) */
+    uint64_t root_tasks;      /* root edges the grid claimed */
+    int32_t grid_waves;       /* persistent waves launched */
     int32_t flags_used;
+    double list_ms;           /* HIP-event time inside the list-engine launches (enumerate kernel) */
+    double bitmap_ms;         /* HIP-event time inside the local-bitmap clique engine launches */
+    uint64_t bitmap_alg_bytes;/* the bitmap engine's share of alg_bytes */
 } gm_stats;
 
 /* gm_count flags */
@@ -154,7 +158,8 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
 
 /* The same batch count captured once into a hipGraph (counter resets by
  * kernels, then the fused persistent launch) and replayed by gm_batch_run.
- * Pattern-rule plans only.  The graph and plans must outlive the batch; the
+ * Pattern-rule plans only; the tuning (root_stride included) is fixed at
+ * creation.  The graph and plans must outlive the batch; the
  * batch owns its scratch, so runs of different batches do not share state. */
 typedef struct gm_batch gm_batch;
 int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng,

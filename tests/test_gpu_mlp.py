@@ -118,7 +118,9 @@ def test_training_reduces_loss_and_ranks_within_groups():
 def test_cost_model_save_load_and_selection(tmp_path):
     """A cost model trained on recorded timings round-trips through its file,
     and whatever it picks counts the same as the analytic pick."""
-    g = N.Graph.rmat(12, 16, 5)
+    # R-MAT 2^10, edge factor 4: every candidate of these patterns counts in
+    # milliseconds (on R-MAT 2^12 ef16 one 5-cycle count is ~1 TB of set work).
+    g = N.Graph.rmat(10, 4, 5)
     v, e = g.csr()
     v, e = v.copy(), e.copy()
     inf = g.info()
@@ -129,6 +131,7 @@ def test_cost_model_save_load_and_selection(tmp_path):
         cands = N.schedule_candidates(n, adj, inf["v_cnt"], len(e), inf["tri_cnt"])
         x = N.encode_candidates(n, adj, v, inf["tri_cnt"])
         ms = []
+        print(f"timing 24 candidates of {adj}", flush=True)
         for c in cands[:24]:
             s = N.Schedule.from_candidate(n, adj, c)
             _, st = N.count(dg, N.DevicePlan(s.plan_desc()))

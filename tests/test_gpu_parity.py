@@ -42,6 +42,10 @@ def gpu_vs_oracle(v, e, sched, mode=N.GM_MODE_PATTERN, waves_per_cu=0):
         oraw_l, ob_l = O.count_clique(csr, desc)
         assert (raw_l, st_l.alg_bytes) == (oraw_l, ob_l)
         assert raw_l == raw
+        # the bitmap engine's share of the tally and time; none without it
+        assert st.bitmap_alg_bytes <= st.alg_bytes
+        assert st_l.bitmap_alg_bytes == 0 and st_l.bitmap_ms == 0.0
+        assert st.list_ms + st.bitmap_ms <= st.kernel_ms * 1.001 + 0.01
     else:
         raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, waves_per_cu=waves_per_cu)
         oraw, ob = O.count_edge_rooted(csr, desc)
@@ -357,9 +361,18 @@ def test_graph_captured_batch_replays():
         again, _ = N.count_batch(dg, eoff, plans)  # eager work between replays
         assert again == eager
     b.free()
-    # a root-edge sub-range (the autotuner's sample): graphs 2..5 only
+    # a root-edge sub-range: graphs 2..5 only
     sub, _ = N.count_batch(dg, eoff[2:7], plans)
     assert sub == eager[2:6]
+    # root sampling: the captured batch, the eager batch and per-graph
+    # launches count the same sample
+    sampled, _ = N.count_batch(dg, eoff, plans, root_stride=5)
+    bs = N.Batch(dg, eoff, plans, root_stride=5)
+    for _ in range(2):
+        assert bs.run()[0] == sampled
+    bs.free()
+    for i in range(10):
+        assert N.count_batch(dg, eoff[i:i + 2], plans, root_stride=5)[0][0] == sampled[i]
 
 
 def test_autotuner_choice_keeps_counts(tmp_path):
@@ -385,20 +398,23 @@ def test_autotuner_choice_keeps_counts(tmp_path):
 
 
 @pytest.mark.parametrize("stride", [2, 7, 64])
-def test_root_stride_phases_partition_the_count(stride):
-    """Root-edge sampling: the sums over the residues r of edges
-    r, r + S, r + 2S, ... add up to the full count exactly."""
-    g = N.Graph.rmat(11, 8, 21)
+def test_root_stride_counts_exactly_the_multiples(stride):
+    """Root-edge sampling counts exactly the edges e with e % stride == 0:
+    equal to the sum of single-edge ranges over those edges, and unchanged
+    when the range is split into pieces."""
+    g = N.Graph.rmat(9, 4, 21)
     v, e = g.csr()
     v, e = v.copy(), e.copy()
     inf = g.info()
     dg = N.DeviceGraph(v, e)
     s = N.Schedule.build(5, PATTERNS["house"][1], inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
     dp = N.DevicePlan(s.plan_desc())
+    nnz = len(e)
+    sampled, _ = N.count_batch(dg, [0, nnz], [dp], root_stride=stride)
+    singles = sum(N.count_batch(dg, [x, x + 1], [dp])[0][0][0] for x in range(0, nnz, stride))
+    assert sampled[0][0] == singles
+    cuts = [0, nnz // 3 + 1, nnz // 2 + 5, nnz]
+    pieces, _ = N.count_batch(dg, cuts, [dp], root_stride=stride)
+    assert sum(p[0] for p in pieces) == sampled[0][0]
     full, _ = N.count(dg, dp)
-    parts = []
-    for r in range(stride):
-        got, _ = N.count_batch(dg, [r, len(e)], [dp], root_stride=stride)
-        parts.append(got[0][0])
-    assert sum(parts) == full
-    assert all(x <= full for x in parts)
+    assert N.count_batch(dg, [0, nnz], [dp], root_stride=1)[0][0][0] == full
