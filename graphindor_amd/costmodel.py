@@ -68,9 +68,12 @@ def _time_plan(dg, nnz, dp, stride, reps):
     return best
 
 
-def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=None, log=None):
+def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=None, log=None, budget_s=30.0):
     """Times candidates of one pattern on one device graph on a shared root
-    sample.  Returns (indices, ms, stride); the analytic pick is always in."""
+    sample.  Returns (indices, ms, stride); the analytic pick is always the
+    first.  The sample starts at ~4096 roots and grows (by 8x) until the
+    analytic pick takes at least target_ms / 4; candidates are timed until
+    `budget_s` is spent."""
     analytic = min(range(len(cands)), key=lambda i: (cands[i].analytic_cost, i))
     idx = list(range(len(cands)))
     if max_cands and len(idx) > max_cands:
@@ -80,17 +83,21 @@ def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=
     else:
         idx.remove(analytic)
         idx = [analytic] + idx
-    # Stride from a first run of the analytic pick on ~64K roots.
-    s0 = max(1, nnz // 65536)
     sched = N.Schedule.from_candidate(n, adj, cands[analytic])
     dp = N.DevicePlan(sched.plan_desc())
-    t0 = _time_plan(dg, nnz, dp, s0, 1)
-    stride = max(1, min(nnz // 1024 if nnz >= 1024 else 1, int(round(s0 * max(t0, 1e-3) / target_ms))))
-    ms = []
+    stride = max(1, nnz // 4096)
+    t0 = _time_plan(dg, nnz, dp, stride, 1)
+    while t0 < target_ms / 4 and stride > 1:
+        stride = max(1, stride // 8)
+        t0 = _time_plan(dg, nnz, dp, stride, 1)
+    ms, start = [], time.time()
     for i in idx:
+        if ms and time.time() - start > budget_s:
+            break
         sched = N.Schedule.from_candidate(n, adj, cands[i])
         dp = N.DevicePlan(sched.plan_desc())
         ms.append(_time_plan(dg, nnz, dp, stride, reps))
+    idx = idx[:len(ms)]
     if log:
         log(f"    {n} {adj}: {len(idx)} candidates, stride {stride}, analytic {ms[0]:.3f} ms, "
             f"best {min(ms):.3f} ms, worst {max(ms):.3f} ms")
