@@ -88,6 +88,60 @@ def cpu_baseline(N, O, v, e, k, target_s=20.0):
                       f"candidates i = 1 mod {stride}: {t:.1f} s measured, scaled x{stride}"}
 
 
+MOTIFS4 = ["0111101111011110", "0111101111001100", "0111101011001000", "0111100010001000", "0110100110010110",
+           "0110100110000100"]
+
+
+def batch_leg(N, count=512, stride=256):
+    """Batched 4-motif counting over the 512-graph batch (graph i from seed
+    5 + i; R-MAT or Erdos-Renyi by coin flip; n log-uniform in [2^10, 2^17];
+    average degree U[4, 64]).  Plain form: one launch per (graph, pattern),
+    as running gpu_mc per graph does (minus its re-uploads).  Batched form:
+    all 6 patterns x 512 graphs in one hipGraph-captured persistent launch
+    with the autotuned tuning.  Both count the same root sample (edges e with
+    e % stride == 0; the full batch is ~10^2 TB of set work) and must agree
+    on every (graph, pattern) sum."""
+    t0 = time.time()
+    g, voff, eoff = N.Graph.batch(count, 5, 10.0, 17.0, 4.0, 64.0)
+    inf = g.info()
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    gen_s = time.time() - t0
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in MOTIFS4]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    # plain: warm once, then time every (graph, pattern) launch
+    N.count_batch(dg, eoff[:2], plans[:1], root_stride=stride)
+    plain = [[0] * len(plans) for _ in range(count)]
+    t0 = time.perf_counter()
+    for i in range(count):
+        for j, p in enumerate(plans):
+            r, _ = N.count_batch(dg, eoff[i:i + 2], [p], root_stride=stride)
+            plain[i][j] = r[0][0]
+    plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tc = N.autotune_batch(dg, eoff, plans, "motifs4:" + N.degree_class(v), None, root_stride=stride)
+    tune_s = time.perf_counter() - t0
+    tun = dict(waves_per_cu=tc["waves_per_cu"], chunk_edges=tc["chunk_edges"], lane_tail=tc["lane_tail"])
+    b = N.Batch(dg, eoff, plans, root_stride=stride, **tun)
+    b.run()
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        got, st = b.run()
+        walls.append(time.perf_counter() - t0)
+        assert got == plain, "batched and per-graph sums differ"
+    b.free()
+    batched_s = min(walls)
+    log(f"batch: {count} graphs, {len(e)} CSR entries (gen {gen_s:.1f} s), stride {stride}: plain {plain_s:.3f} s, "
+        f"batched {batched_s:.3f} s, tune {tune_s:.2f} s {tc}")
+    return {"graphs": count, "csr_entries": int(len(e)), "root_sample": f"edges e with e % {stride} == 0",
+            "plain_per_pattern_launches_s": round(plain_s, 4), "launches_plain": count * len(plans),
+            "batched_hipgraph_s": round(batched_s, 4), "speedup": round(plain_s / batched_s, 3),
+            "autotune": {k: tc[k] for k in ("waves_per_cu", "chunk_edges", "lane_tail", "tried")},
+            "autotune_s": round(tune_s, 3), "sums_equal": True}
+
+
 def calibration_copy(mb):
     """One device-to-device copy of `mb` MiB (HIP's own copy kernel), so that
     a PMC run sees a kernel of known traffic next to the counting kernels."""
@@ -114,6 +168,8 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-control", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched motif leg")
+    ap.add_argument("--batch-stride", type=int, default=256)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
@@ -219,6 +275,8 @@ def main():
                      "region_ms": round(r5["kernel_ms"], 3),
                      "other_kernel_ms": round(r5["kernel_ms"] - dom["ms"], 3)},
     }
+    if not args.no_batch:
+        out["config"]["batch_4motif"] = batch_leg(N, 512, args.batch_stride)
     if not args.no_cpu:
         from oracle import oracle as O
         out["cpu_baseline"] = cpu_baseline(N, O, iv.copy(), ie.copy(), 5)

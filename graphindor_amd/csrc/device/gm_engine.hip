@@ -395,21 +395,29 @@ __device__ void clique_tail_count(Wave& w, SetRef L) {
 
 
 // ---------------------------------------------------------------------------
-// Lane-parallel last candidate level of the pattern rule.
+// Last candidate level of the pattern rule, lane-parallel where it pays.
 //
 // At depth D = n-k-1 (IEP tail of k >= 2 vertices) or D = n-2 (no IEP) the
 // sets the schedule builds for a candidate v are only read for their sizes
 // and for membership of the embedded vertices: every prefix q of last[D]
 // is S_q = F_q ∩ N(v) with F_q (its father) fixed above D, or N(v) itself.
-// So each lane takes its own candidate and computes |S_q|, the break-size
-// prune, and then either the IEP terminal of depth D+1
+// So a candidate's contribution is |S_q| for each q, the break-size prune,
+// and then either the IEP terminal of depth D+1
 // (gpu/function/pattern_matching.cuh:110-139) or the restricted last level
-// (pattern_matching.cuh:240-257) for it, without materialising anything.
-// The fathers are staged in LDS when they fit.  Byte model: as the wave
-// path, except that the builds of depth D are charged as size-only.
+// (pattern_matching.cuh:240-257), with nothing materialised.
+//
+// Light candidates -- whose intersections walk at most kLaneTailWork
+// elements in all -- are evaluated one per lane, each lane iterating the
+// shorter list and searching the longer.  Heavy ones (hub neighbourhoods)
+// would hold the whole wave up while one lane walks thousands of elements,
+// so they are evaluated one at a time by the whole wave, 64 elements per
+// step.  Both evaluate the same arithmetic; the fathers are staged in LDS
+// when they fit.  Byte model: as the wave path, except that the builds of
+// depth D are charged as size-only.
 // ---------------------------------------------------------------------------
 constexpr int kTailMaxPrefix = 4;
 constexpr int kTailMaxIep = 8;
+constexpr int32_t kLaneTailWork = 64;
 
 // |F ∩ N| over the elements below `limit`, iterating the shorter list (per lane).
 __device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
@@ -431,31 +439,179 @@ __device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t n
     return c;
 }
 
+// The same count by the whole wave (uniform arguments and result): lane i
+// takes element i of each 64-element step of the shorter list.
+__device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
+                                                    uint32_t limit) {
+    const uint32_t* s = F;
+    const uint32_t* l = Nv;
+    int32_t ns = nF, nl = nN;
+    if (nF > nN) {
+        s = Nv;
+        l = F;
+        ns = nN;
+        nl = nF;
+    }
+    const int32_t lim = (limit == 0xffffffffu) ? ns : lower_bound_u(s, ns, limit);
+    int32_t c = 0;
+    for (int32_t b = 0; b < lim; b += kWave) {
+        const int32_t i = b + lane_id();
+        const bool hit = i < lim && contains(l, nl, s[i]);
+        c += (int32_t)__popcll(ballot(hit));
+    }
+    return c;
+}
+
+// Everything the last level reads that does not depend on the candidate.
+struct TailCtx {
+    int32_t pid[kTailMaxPrefix];
+    const uint32_t* fptr[kTailMaxPrefix];
+    int32_t flen[kTailMaxPrefix];
+    bool alias[kTailMaxPrefix];
+    int32_t base[kTailMaxIep];
+    const uint32_t* sptr[kTailMaxIep];
+    int32_t slen[kTailMaxIep];
+    int32_t at[kTailMaxIep];  // position of the IEP row's prefix in pid[], or -1
+    int qat;                  // position of the last loop set in pid[] (no IEP), or -1
+    SetRef qset;              // the last loop set when it was built above D
+};
+
+// Contribution of candidate v (not embedded, below the bound) to the sum,
+// and its byte charge beyond the 12 bytes of reading v and its row.  WAVE:
+// every lane holds the same v and the counts are wave-wide; otherwise each
+// lane evaluates its own v.
+template <int D, bool WAVE>
+__device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, uint32_t v, const uint32_t* Nv,
+                                               int32_t dv, unsigned long long& acc, unsigned long long& bytes) {
+    const DevPlan& p = *w.pl;
+    auto count = [&](const uint32_t* F, int32_t nF, const uint32_t* N2, int32_t n2, uint32_t limit) -> int32_t {
+        if constexpr (WAVE) return wave_isect_count(F, nF, N2, n2, limit);
+        else return lane_isect_count(F, nF, N2, n2, limit);
+    };
+    int32_t sz[kTailMaxPrefix];
+#pragma unroll
+    for (int j = 0; j < kTailMaxPrefix; ++j) {
+        sz[j] = 0;
+        if (c.pid[j] == -1) continue;
+        if (c.alias[j]) {
+            sz[j] = dv;
+        } else {
+            sz[j] = count(c.fptr[j], c.flen[j], Nv, dv, 0xffffffffu);
+            bytes += 4ull * (unsigned long long)(c.flen[j] + dv);
+        }
+        if (sz[j] == p.brk[c.pid[j]]) return;
+    }
+    // x in S_j  <=>  x in N(v) and (alias or x in F_j).
+    auto member = [&](int j, uint32_t x) -> bool {
+        bool in_n = contains(Nv, dv, x);
+        bool in_f = true;
+#pragma unroll
+        for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+            if (jj == j && !c.alias[jj]) in_f = contains(c.fptr[jj], c.flen[jj], x);
+        return in_n && in_f;
+    };
+    if (p.k >= 2) {
+        int32_t ans[kTailMaxIep];
+#pragma unroll
+        for (int e = 0; e < kTailMaxIep; ++e) {
+            ans[e] = 0;
+            if (e >= p.iep_vn) continue;
+            const int j = c.at[e];
+            if (j >= 0) {
+                int32_t szj = 0;
+#pragma unroll
+                for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+                    if (jj == j) szj = sz[jj];
+                if (p.iep_vflag[e]) {
+                    ans[e] = szj - p.iep_vcoef[e];
+                } else {
+                    int32_t cnt = szj;
+#pragma unroll
+                    for (int x = 0; x < D; ++x) cnt -= member(j, w.emb[x]) ? 1 : 0;
+                    ans[e] = cnt;
+                    bytes += 4ull * (D + 1);
+                }
+            } else if (p.iep_vflag[e]) {
+                ans[e] = c.base[e];
+            } else {
+                ans[e] = c.base[e] - (contains(c.sptr[e], c.slen[e], v) ? 1 : 0);
+                bytes += 4ull * (D + 1);
+            }
+        }
+        int last_pos = -1;
+        long long val = 0;
+        for (int pos = 0; pos < p.iep_an; ++pos) {
+            const int ap = p.iep_pos[pos];
+            long long x = 0;
+#pragma unroll
+            for (int e = 0; e < kTailMaxIep; ++e)
+                if (e == ap) x = ans[e];
+            if (pos == last_pos + 1) val = x;
+            else if (val != 0) val = (long long)((unsigned long long)val * (unsigned long long)x);
+            if (p.iep_flag[pos]) {
+                last_pos = pos;
+                acc += (unsigned long long)val * (unsigned long long)(long long)p.iep_coef[pos];
+            }
+        }
+    } else {
+        // Depth D+1 = n-1: |{x in S : x < bound'} \ emb'| with emb' = emb + v.
+        uint32_t b1 = 0xffffffffu;
+        for (int rr = p.rlast[D + 1]; rr != -1; rr = p.rnext[rr]) {
+            const int idx = p.rindex[rr];
+            uint32_t ev = (idx == D) ? v : emb_at<D>(w, idx);
+            b1 = ev < b1 ? ev : b1;
+        }
+        int32_t res;
+        if (c.qat >= 0) {
+            bool al = false;
+            const uint32_t* F = nullptr;
+            int32_t nF = 0;
+#pragma unroll
+            for (int jj = 0; jj < kTailMaxPrefix; ++jj)
+                if (jj == c.qat) {
+                    al = c.alias[jj];
+                    F = c.fptr[jj];
+                    nF = c.flen[jj];
+                }
+            res = al ? lower_bound_u(Nv, dv, b1) : count(F, nF, Nv, dv, b1);
+#pragma unroll
+            for (int x = 0; x < D; ++x)
+                if (w.emb[x] < b1 && member(c.qat, w.emb[x])) --res;
+        } else {
+            const int32_t lim = lower_bound_u(c.qset.ptr, c.qset.len, b1);
+            res = lim;
+#pragma unroll
+            for (int x = 0; x < D; ++x)
+                if (contains(c.qset.ptr, lim, w.emb[x])) --res;
+            if (contains(c.qset.ptr, lim, v)) --res;
+        }
+        acc += (unsigned long long)(long long)res;
+        bytes += 4ull * (D + 1);
+    }
+}
+
 template <int D>
 __device__ void pattern_tail(Wave& w, SetRef loop) {
     const DevPlan& p = *w.pl;
     const int ln = lane_id();
     const uint32_t bound = restriction_bound<D>(w);
-    // Prefixes of depth D, their fathers (staged in LDS when they fit).
-    int32_t pid[kTailMaxPrefix];
-    const uint32_t* fptr[kTailMaxPrefix];
-    int32_t flen[kTailMaxPrefix];
-    bool alias[kTailMaxPrefix];
+    TailCtx c;
+    // Prefixes of depth D and their fathers (staged in LDS when they fit).
     int q = p.last[D];
     int32_t staged = 0;
 #pragma unroll
     for (int j = 0; j < kTailMaxPrefix; ++j) {
-        pid[j] = q;
-        alias[j] = true;
-        fptr[j] = nullptr;
-        flen[j] = 0;
+        c.pid[j] = q;
+        c.alias[j] = true;
+        c.fptr[j] = nullptr;
+        c.flen[j] = 0;
         if (q != -1) {
             int f = p.father[q];
             if (f >= 0) {
                 SetRef fs = get_set(w, f);
-                alias[j] = false;
-                fptr[j] = fs.ptr;
-                flen[j] = fs.len;
+                c.alias[j] = false;
+                c.fptr[j] = fs.ptr;
+                c.flen[j] = fs.len;
                 staged += fs.len;
             }
             q = p.next[q];
@@ -465,48 +621,44 @@ __device__ void pattern_tail(Wave& w, SetRef loop) {
         int32_t off = 0;
 #pragma unroll
         for (int j = 0; j < kTailMaxPrefix; ++j) {
-            if (pid[j] == -1 || alias[j]) continue;
-            for (int32_t t = ln; t < flen[j]; t += kWave) w.stage[off + t] = fptr[j][t];
-            fptr[j] = w.stage + off;
-            off += flen[j];
+            if (c.pid[j] == -1 || c.alias[j]) continue;
+            for (int32_t t = ln; t < c.flen[j]; t += kWave) w.stage[off + t] = c.fptr[j][t];
+            c.fptr[j] = w.stage + off;
+            off += c.flen[j];
         }
         wave_sync();
     }
     // IEP rows on sets built above D: |S| - coef, or |S \ emb[0:D)| (uniform).
-    int32_t base[kTailMaxIep];
-    const uint32_t* sptr[kTailMaxIep];
-    int32_t slen[kTailMaxIep];
-    int32_t at[kTailMaxIep];  // position of the row's prefix in pid[], or -1
 #pragma unroll
     for (int e = 0; e < kTailMaxIep; ++e) {
-        base[e] = 0;
-        sptr[e] = nullptr;
-        slen[e] = 0;
-        at[e] = -1;
+        c.base[e] = 0;
+        c.sptr[e] = nullptr;
+        c.slen[e] = 0;
+        c.at[e] = -1;
         if (p.k >= 2 && e < p.iep_vn) {
             int ip = p.iep_id[e];
 #pragma unroll
             for (int j = 0; j < kTailMaxPrefix; ++j)
-                if (pid[j] == ip) at[e] = j;
-            if (at[e] < 0) {
+                if (c.pid[j] == ip) c.at[e] = j;
+            if (c.at[e] < 0) {
                 SetRef s = get_set(w, ip);
-                sptr[e] = s.ptr;
-                slen[e] = s.len;
-                base[e] = p.iep_vflag[e] ? s.len - p.iep_vcoef[e] : subtract_emb<D>(w, s.ptr, s.len);
+                c.sptr[e] = s.ptr;
+                c.slen[e] = s.len;
+                c.base[e] = p.iep_vflag[e] ? s.len - p.iep_vcoef[e] : subtract_emb<D>(w, s.ptr, s.len);
             }
         }
     }
     // Last loop set when there is no IEP tail.
-    int qlast = -1, qat = -1;
-    SetRef qset;
-    qset.ptr = nullptr;
-    qset.len = 0;
+    c.qat = -1;
+    c.qset.ptr = nullptr;
+    c.qset.len = 0;
+    c.qset.pad = 0;
     if (p.k == 0) {
-        qlast = p.loopset[D + 1];
+        const int qlast = p.loopset[D + 1];
 #pragma unroll
         for (int j = 0; j < kTailMaxPrefix; ++j)
-            if (pid[j] == qlast) qat = j;
-        if (qat < 0) qset = get_set(w, qlast);
+            if (c.pid[j] == qlast) c.qat = j;
+        if (c.qat < 0) c.qset = get_set(w, qlast);
     }
 
     unsigned long long acc = 0, bytes = 0;
@@ -516,116 +668,33 @@ __device__ void pattern_tail(Wave& w, SetRef loop) {
         const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
         const bool live = in_range && v < bound;
         const bool stop_after = ballot(in_range && v >= bound) != 0;
+        bool heavy = false;
         if (live) {
             bytes += 4ull;
             if (!in_emb<D>(w, v)) {
                 bytes += 8ull;
                 const int64_t l = w.a.row[v], r = w.a.row[v + 1];
-                const uint32_t* Nv = w.a.col + l;
                 const int32_t dv = (int32_t)(r - l);
-                int32_t sz[kTailMaxPrefix];
-                bool pruned = false;
+                int32_t work = 0;
 #pragma unroll
-                for (int j = 0; j < kTailMaxPrefix; ++j) {
-                    sz[j] = 0;
-                    if (pid[j] == -1 || pruned) continue;
-                    if (alias[j]) {
-                        sz[j] = dv;
-                    } else {
-                        sz[j] = lane_isect_count(fptr[j], flen[j], Nv, dv, 0xffffffffu);
-                        bytes += 4ull * (unsigned long long)(flen[j] + dv);
-                    }
-                    if (sz[j] == p.brk[pid[j]]) pruned = true;
-                }
-                if (!pruned) {
-                    // x in S_j  <=>  x in N(v) and (alias or x in F_j).
-                    auto member = [&](int j, uint32_t x) -> bool {
-                        bool in_n = contains(Nv, dv, x);
-                        bool in_f = true;
-#pragma unroll
-                        for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-                            if (jj == j && !alias[jj]) in_f = contains(fptr[jj], flen[jj], x);
-                        return in_n && in_f;
-                    };
-                    if (p.k >= 2) {
-                        int32_t ans[kTailMaxIep];
-#pragma unroll
-                        for (int e = 0; e < kTailMaxIep; ++e) {
-                            ans[e] = 0;
-                            if (e >= p.iep_vn) continue;
-                            const int j = at[e];
-                            if (j >= 0) {
-                                int32_t szj = 0;
-#pragma unroll
-                                for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-                                    if (jj == j) szj = sz[jj];
-                                if (p.iep_vflag[e]) {
-                                    ans[e] = szj - p.iep_vcoef[e];
-                                } else {
-                                    int32_t c = szj;
-#pragma unroll
-                                    for (int x = 0; x < D; ++x) c -= member(j, w.emb[x]) ? 1 : 0;
-                                    ans[e] = c;
-                                    bytes += 4ull * (D + 1);
-                                }
-                            } else if (p.iep_vflag[e]) {
-                                ans[e] = base[e];
-                            } else {
-                                ans[e] = base[e] - (contains(sptr[e], slen[e], v) ? 1 : 0);
-                                bytes += 4ull * (D + 1);
-                            }
-                        }
-                        int last_pos = -1;
-                        long long val = 0;
-                        for (int pos = 0; pos < p.iep_an; ++pos) {
-                            const int ap = p.iep_pos[pos];
-                            long long x = 0;
-#pragma unroll
-                            for (int e = 0; e < kTailMaxIep; ++e)
-                                if (e == ap) x = ans[e];
-                            if (pos == last_pos + 1) val = x;
-                            else if (val != 0) val = (long long)((unsigned long long)val * (unsigned long long)x);
-                            if (p.iep_flag[pos]) {
-                                last_pos = pos;
-                                acc += (unsigned long long)val * (unsigned long long)(long long)p.iep_coef[pos];
-                            }
-                        }
-                    } else {
-                        // Depth D+1 = n-1: |{x in S : x < bound'} \ emb'| with emb' = emb + v.
-                        uint32_t b1 = 0xffffffffu;
-                        for (int rr = p.rlast[D + 1]; rr != -1; rr = p.rnext[rr]) {
-                            const int idx = p.rindex[rr];
-                            uint32_t ev = (idx == D) ? v : emb_at<D>(w, idx);
-                            b1 = ev < b1 ? ev : b1;
-                        }
-                        int32_t res;
-                        if (qat >= 0) {
-                            bool al = false;
-                            const uint32_t* F = nullptr;
-                            int32_t nF = 0;
-#pragma unroll
-                            for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-                                if (jj == qat) {
-                                    al = alias[jj];
-                                    F = fptr[jj];
-                                    nF = flen[jj];
-                                }
-                            res = al ? lower_bound_u(Nv, dv, b1) : lane_isect_count(F, nF, Nv, dv, b1);
-#pragma unroll
-                            for (int x = 0; x < D; ++x)
-                                if (w.emb[x] < b1 && member(qat, w.emb[x])) --res;
-                        } else {
-                            const int32_t lim = lower_bound_u(qset.ptr, qset.len, b1);
-                            res = lim;
-#pragma unroll
-                            for (int x = 0; x < D; ++x)
-                                if (contains(qset.ptr, lim, w.emb[x])) --res;
-                            if (contains(qset.ptr, lim, v)) --res;
-                        }
-                        acc += (unsigned long long)(long long)res;
-                        bytes += 4ull * (D + 1);
-                    }
-                }
+                for (int j = 0; j < kTailMaxPrefix; ++j)
+                    if (c.pid[j] != -1 && !c.alias[j]) work += c.flen[j] < dv ? c.flen[j] : dv;
+                heavy = work > kLaneTailWork;
+                if (!heavy) tail_candidate<D, false>(w, c, v, w.a.col + l, dv, acc, bytes);
+            }
+        }
+        // Heavy candidates, one at a time, wave-wide.
+        uint64_t hm = ballot(heavy);
+        while (hm != 0) {
+            const int j = __ffsll((unsigned long long)hm) - 1;
+            hm &= hm - 1;
+            const uint32_t hv = rdlane(v, j);
+            const int64_t l = w.a.row[hv], r = w.a.row[hv + 1];
+            unsigned long long a2 = 0, b2 = 0;
+            tail_candidate<D, true>(w, c, hv, w.a.col + l, (int32_t)(r - l), a2, b2);
+            if (ln == 0) {
+                acc += a2;
+                bytes += b2;
             }
         }
         if (stop_after) break;
