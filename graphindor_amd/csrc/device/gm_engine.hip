@@ -59,9 +59,28 @@ struct SetRef {
 // Elements of a clique-mode tail loop set staged in LDS per wave.
 constexpr int kStage = 2048;
 
+// Pattern-rule last level (pattern_tail): prefixes and IEP rows it handles.
+constexpr int kTailMaxPrefix = 4;
+constexpr int kTailMaxIep = 8;
+
+// Everything the last level reads that does not depend on the candidate.
+struct TailCtx {
+    int32_t pid[kTailMaxPrefix];
+    const uint32_t* fptr[kTailMaxPrefix];
+    int32_t flen[kTailMaxPrefix];
+    bool alias[kTailMaxPrefix];
+    int32_t base[kTailMaxIep];
+    const uint32_t* sptr[kTailMaxIep];
+    int32_t slen[kTailMaxIep];
+    int32_t at[kTailMaxIep];  // position of the IEP row's prefix in pid[], or -1
+    int qat;                  // position of the last loop set in pid[] (no IEP), or -1
+    SetRef qset;              // the last loop set when it was built above D
+};
+
 struct WaveLds {
     SetRef sets[kMaxSets];
     int32_t ans[GM_MAX_IEP];
+    TailCtx tail;  // pattern_tail's candidate-independent state (LDS, not registers)
 };
 
 template <int MODE>
@@ -90,7 +109,8 @@ struct KernelArgs {
     int32_t ng;
     unsigned long long* sums;     // [ng * np] raw sums
     int32_t tail_off;             // 1: never use the lane-parallel pattern tail
-    int32_t chunk_mult;           // root edges per task = 64 * chunk_mult
+    int32_t tail_work;            // pattern tail: candidates walking more elements than this go wave-wide
+    int32_t task_max;             // most work items one claim takes
     int64_t root_begin, root_end; // root edges counted: [gedge[0], gedge[ng])
     int64_t root_stride;          // counted root edges: root_begin + i * root_stride < root_end (root_begin is
                                   // the range's first multiple of root_stride)
@@ -99,7 +119,6 @@ struct KernelArgs {
     int64_t wave_stride;  // uint32 per wave
     int32_t cap;          // capacity of a prefix slot
     int32_t cap_vi;       // capacity of a vertex-induced slot
-    int64_t nchunks;
     Counters* ctr;
     int32_t stats;
     int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
@@ -206,7 +225,7 @@ __device__ __forceinline__ bool in_emb(const Wave& w, uint32_t v) {
 }
 
 // Wave-cooperative |a ∩ b|, compacted into out (out == nullptr: size only).
-__device__ int32_t intersect(Wave& w, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* out,
+__device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* out,
                              int32_t cap) {
     if (na > nb) {
         const uint32_t* t = a;
@@ -285,7 +304,7 @@ __device__ __forceinline__ uint32_t restriction_bound(const Wave& w) {
 // loop-set elements adjacent to no embedded vertex the pattern marks as a
 // non-neighbour of vertex D.
 template <int D>
-__device__ SetRef remove_anti_edge(Wave& w, SetRef in) {
+__device__ __forceinline__ SetRef remove_anti_edge(Wave& w, SetRef in) {
     const DevPlan& p = *w.pl;
     uint32_t* out = w.scratch + (int64_t)p.P * w.a.cap + (int64_t)D * w.a.cap_vi;
     const int ln = lane_id();
@@ -323,7 +342,7 @@ __device__ SetRef remove_anti_edge(Wave& w, SetRef in) {
 
 // IEP terminal (gpu/function/pattern_matching.cuh:110-139).
 template <int D>
-__device__ void iep_terminal(Wave& w) {
+__device__ __forceinline__ void iep_terminal(Wave& w) {
     const DevPlan& p = *w.pl;
     for (int i = 0; i < p.iep_vn; ++i) {
         SetRef s = get_set(w, p.iep_id[i]);
@@ -369,7 +388,7 @@ __device__ __forceinline__ bool lds_contains(const uint32_t* b, int32_t nb, uint
 // other.  Per candidate the byte model charges what the wave path does:
 // 12 B (element + row pointers) + 4 (|L| + |N(v)|) for a size-only
 // intersection.
-__device__ void clique_tail_count(Wave& w, SetRef L) {
+__device__ __forceinline__ void clique_tail_count(Wave& w, SetRef L) {
     const int ln = lane_id();
     for (int32_t i = ln; i < L.len; i += kWave) w.stage[i] = L.ptr[i];
     wave_sync();
@@ -415,9 +434,7 @@ __device__ void clique_tail_count(Wave& w, SetRef L) {
 // when they fit.  Byte model: as the wave path, except that the builds of
 // depth D are charged as size-only.
 // ---------------------------------------------------------------------------
-constexpr int kTailMaxPrefix = 4;
-constexpr int kTailMaxIep = 8;
-constexpr int32_t kLaneTailWork = 64;
+constexpr int32_t kLaneTailWork = 64;  // default light/heavy bound (gm_tuning.lane_tail > 0 sets it)
 
 // |F ∩ N| over the elements below `limit`, iterating the shorter list (per lane).
 __device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
@@ -462,19 +479,6 @@ __device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t n
     return c;
 }
 
-// Everything the last level reads that does not depend on the candidate.
-struct TailCtx {
-    int32_t pid[kTailMaxPrefix];
-    const uint32_t* fptr[kTailMaxPrefix];
-    int32_t flen[kTailMaxPrefix];
-    bool alias[kTailMaxPrefix];
-    int32_t base[kTailMaxIep];
-    const uint32_t* sptr[kTailMaxIep];
-    int32_t slen[kTailMaxIep];
-    int32_t at[kTailMaxIep];  // position of the IEP row's prefix in pid[], or -1
-    int qat;                  // position of the last loop set in pid[] (no IEP), or -1
-    SetRef qset;              // the last loop set when it was built above D
-};
 
 // Contribution of candidate v (not embedded, below the bound) to the sum,
 // and its byte charge beyond the 12 bytes of reading v and its row.  WAVE:
@@ -591,11 +595,14 @@ __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, 
 }
 
 template <int D>
-__device__ void pattern_tail(Wave& w, SetRef loop) {
+__device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound);
+
+template <int D>
+__device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
     const DevPlan& p = *w.pl;
     const int ln = lane_id();
     const uint32_t bound = restriction_bound<D>(w);
-    TailCtx c;
+    TailCtx c;  // built here, then kept in LDS for the candidate loop
     // Prefixes of depth D and their fathers (staged in LDS when they fit).
     int q = p.last[D];
     int32_t staged = 0;
@@ -660,7 +667,15 @@ __device__ void pattern_tail(Wave& w, SetRef loop) {
             if (c.pid[j] == qlast) c.qat = j;
         if (c.qat < 0) c.qset = get_set(w, qlast);
     }
+    if (ln == 0) w.lds->tail = c;
+    wave_sync();
+    return pattern_tail_loop<D>(w, loop, bound);
+}
 
+template <int D>
+__device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound) {
+    const int ln = lane_id();
+    const TailCtx& c = w.lds->tail;
     unsigned long long acc = 0, bytes = 0;
     for (int32_t b0 = 0; b0 < loop.len; b0 += kWave) {
         const int32_t i = b0 + ln;
@@ -679,7 +694,7 @@ __device__ void pattern_tail(Wave& w, SetRef loop) {
 #pragma unroll
                 for (int j = 0; j < kTailMaxPrefix; ++j)
                     if (c.pid[j] != -1 && !c.alias[j]) work += c.flen[j] < dv ? c.flen[j] : dv;
-                heavy = work > kLaneTailWork;
+                heavy = work > w.a.tail_work;
                 if (!heavy) tail_candidate<D, false>(w, c, v, w.a.col + l, dv, acc, bytes);
             }
         }
@@ -704,9 +719,9 @@ __device__ void pattern_tail(Wave& w, SetRef loop) {
     wave_sync();
 }
 
-template <int D, int MODE>
-__device__ void dfs(Wave& w) {
-    if constexpr (D >= GM_MAX_PATTERN) {
+template <int D, int MODE, int NMAX>
+__device__ __forceinline__ void dfs(Wave& w) {
+    if constexpr (D >= NMAX) {
         return;
     } else {
         const DevPlan& p = *w.pl;
@@ -765,7 +780,7 @@ __device__ void dfs(Wave& w) {
                 }
                 if (prune) continue;
                 w.emb[D] = v;
-                dfs<D + 1, MODE>(w);
+                dfs<D + 1, MODE, NMAX>(w);
             }
         }
     }
@@ -782,8 +797,8 @@ __device__ __forceinline__ int32_t edge_source(const int64_t* row, int32_t nv, i
     return base;
 }
 
-template <int MODE>
-__device__ void process_root(Wave& w, uint32_t v0, uint32_t v1) {
+template <int MODE, int NMAX>
+__device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1) {
     const DevPlan& p = *w.pl;
     int64_t l0 = w.a.row[v0], r0 = w.a.row[v0 + 1];
     for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0));
@@ -794,7 +809,7 @@ __device__ void process_root(Wave& w, uint32_t v0, uint32_t v1) {
     }
     w.emb[0] = v0;
     w.emb[1] = v1;
-    dfs<2, MODE>(w);
+    dfs<2, MODE, NMAX>(w);
 }
 
 // Sum of a per-lane value over the wave (all lanes receive it).
@@ -831,7 +846,10 @@ __device__ __forceinline__ void flush_sum(Wave& w, int32_t g, int32_t pi) {
 // Persistent enumeration over tasks (root chunk c, plan pi), chunk-major so
 // the plans of one chunk run close together and share its neighbour lists
 // in cache.  A root's count goes to the (graph, plan) cell of its edge.
-template <int MODE>
+// NMAX: the largest pattern size among the launch's plans; the DFS is
+// instantiated only that deep, which keeps registers (and so occupancy) to
+// what the patterns need.
+template <int MODE, int NMAX>
 __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     __shared__ WaveShared<MODE> lds_all[kWavesPerBlock];
     Wave w;
@@ -855,33 +873,43 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     }
     wave_sync();
     const int ln = lane_id();
-    const int64_t task_edges = (int64_t)kWave * a.chunk_mult;
+    // Work items are (logical root, plan) pairs, root-major, so the waves
+    // that share a root edge's neighbour lists run its plans close together.
+    // Guided self-scheduling: a claim takes remaining / (4 * waves) items,
+    // at most task_max and at least 1, so large claims amortise the cursor
+    // atomic early and single roots spread over the whole grid at the end
+    // (a hub's root edges can each hold seconds of work).
+    const int64_t nitems = a.nroots * (int64_t)a.np;
+    const int64_t nwaves_total = (int64_t)gridDim.x * kWavesPerBlock;
     unsigned long long roots = 0;
     int32_t cur_g = 0, cur_p = 0;
     while (true) {
-        unsigned long long task = 0;
-        if (ln == 0) task = atomicAdd(&a.ctr->cursor, 1ull);
-        task = (unsigned long long)uni64((int64_t)__shfl((long long)task, 0));
-        if ((long long)task >= a.nchunks) break;
-        const int32_t pi = (int32_t)(task % (unsigned long long)a.np);
-        const int64_t chunk = (int64_t)(task / (unsigned long long)a.np);
-        w.pl = a.plans[pi];
-        const DevPlan& p = *w.pl;
-        w.tail_depth = a.tail_off ? -1 : p.tail_depth;
-        const bool restricted1 = p.rlast[1] != -1;
-        // Logical root indices [i_first, i_end) map to edges root_begin + i * root_stride.
-        const int64_t i_first = chunk * task_edges;
-        const int64_t i_end = i_first + task_edges < a.nroots ? i_first + task_edges : a.nroots;
-        for (int64_t i0 = i_first; i0 < i_end; i0 += kWave) {
-            const int64_t e = a.root_begin + (i0 + ln) * a.root_stride;
-            bool ok = i0 + ln < i_end;
+        unsigned long long start = 0;
+        int64_t take = 0;
+        if (ln == 0) {
+            const unsigned long long seen = __hip_atomic_load(&a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int64_t rem = nitems - (int64_t)seen;
+            take = rem / (4 * nwaves_total);
+            take = take < 1 ? 1 : (take > a.task_max ? a.task_max : take);
+            start = atomicAdd(&a.ctr->cursor, (unsigned long long)take);
+        }
+        start = (unsigned long long)uni64((int64_t)__shfl((long long)start, 0));
+        take = uni64(__shfl((long long)take, 0));
+        if ((int64_t)start >= nitems) break;
+        const int64_t i_end = (int64_t)start + take < nitems ? (int64_t)start + take : nitems;
+        for (int64_t i0 = (int64_t)start; i0 < i_end; i0 += kWave) {
+            const int64_t item = i0 + ln;
+            bool ok = item < i_end;
             uint32_t v0 = 0, v1 = 0;
-            int32_t g = 0;
+            int32_t g = 0, pl = 0;
             if (ok) {
+                const int64_t ri = a.np == 1 ? item : item / a.np;
+                pl = a.np == 1 ? 0 : (int32_t)(item - ri * a.np);
+                const int64_t e = a.root_begin + ri * a.root_stride;
                 v1 = a.col[e];
                 v0 = (uint32_t)edge_source(a.row, a.nv, e);
                 if (a.ng > 1) g = edge_graph(a.gedge, a.ng, e);
-                if (restricted1 && v0 <= v1) ok = false;
+                if (a.plans[pl]->rlast[1] != -1 && v0 <= v1) ok = false;
                 if (MODE == GM_MODE_CLIQUE && a.root_min_outdeg > 0 && a.row[v0 + 1] - a.row[v0] <= a.root_min_outdeg)
                     ok = false;
             }
@@ -891,14 +919,17 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 m &= m - 1;
                 const uint32_t u0 = rdlane(v0, j), u1 = rdlane(v1, j);
                 const int32_t gj = (int32_t)rdlane((uint32_t)g, j);
-                if (gj != cur_g || pi != cur_p) {
+                const int32_t pj = (int32_t)rdlane((uint32_t)pl, j);
+                if (gj != cur_g || pj != cur_p) {
                     flush_sum(w, cur_g, cur_p);
                     cur_g = gj;
-                    cur_p = pi;
+                    cur_p = pj;
                 }
+                w.pl = a.plans[pj];
+                w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
                 ++roots;
                 if (a.stats) w.bytes += 24ull;
-                process_root<MODE>(w, u0, u1);
+                process_root<MODE, NMAX>(w, u0, u1);
             }
         }
     }
@@ -1179,6 +1210,36 @@ int fail(int code, const std::string& msg) {
 
 // Grow-only per-device scratch shared by calls (one host thread at a time
 // per device, as documented for the handles).
+// Kernel instantiation for a launch whose largest pattern has n vertices.
+int nmax_for(int n) { return n <= 4 ? 4 : n; }
+
+template <int MODE>
+hipError_t enum_occupancy(int nmax, int* blocks_per_cu) {
+    switch (nmax) {
+        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 4>, kBlock, 0);
+        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 5>, kBlock, 0);
+        case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 6>, kBlock, 0);
+        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 7>, kBlock, 0);
+    }
+}
+
+template <int MODE>
+void enum_launch(int nmax, int blocks, hipStream_t s, const KernelArgs& a) {
+    switch (nmax) {
+        case 4: hipLaunchKernelGGL((enumerate_kernel<MODE, 4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((enumerate_kernel<MODE, 5>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((enumerate_kernel<MODE, 6>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL((enumerate_kernel<MODE, 7>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    }
+}
+
+// Persistent grid for a kernel: `waves_per_cu` (0 = all the occupancy allows).
+int grid_blocks(int cus, int blocks_per_cu, int waves_per_cu) {
+    blocks_per_cu = std::max(1, blocks_per_cu);
+    const int w = waves_per_cu > 0 ? std::min(waves_per_cu, blocks_per_cu * kWavesPerBlock) : blocks_per_cu * kWavesPerBlock;
+    return std::max(1, cus * w / kWavesPerBlock);
+}
+
 struct DeviceCache {
     std::mutex mu;
     void* scratch = nullptr;
@@ -1501,18 +1562,29 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     std::lock_guard<std::mutex> lock(dc.mu);
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-    // Default grid: every block the occupancy of either kernel allows
+    // Default grids: every block the occupancy of the launched kernel allows
     // (persistent waves; the kernels exit when the task cursor runs out).
-    int occ_pattern = 0, occ_clique = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pattern, enumerate_kernel<GM_MODE_PATTERN>, kBlock, 0));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_clique, enumerate_kernel<GM_MODE_CLIQUE>, kBlock, 0));
-    const int max_blocks_per_cu = std::max(1, std::min(occ_pattern, occ_clique));
-    const int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : max_blocks_per_cu * kWavesPerBlock;
-    const int blocks = std::max(1, cus * std::min(waves_per_cu, max_blocks_per_cu * kWavesPerBlock) / kWavesPerBlock);
-    const int64_t nwaves = (int64_t)blocks * kWavesPerBlock;
+    int nmax_pat = 0, nmax_clq = 0;
+    for (int i = 0; i < np; ++i) {
+        int& m = plans[i]->host.mode == GM_MODE_PATTERN ? nmax_pat : nmax_clq;
+        m = std::max(m, nmax_for(plans[i]->host.n));
+    }
+    const int req_waves = (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0;
+    int blocks_pat = 0, blocks_clq = 0;
+    if (nmax_pat) {
+        int occ = 0;
+        HIP_TRY(enum_occupancy<GM_MODE_PATTERN>(nmax_pat, &occ));
+        blocks_pat = grid_blocks(cus, occ, req_waves);
+    }
+    if (nmax_clq) {
+        int occ = 0;
+        HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, &occ));
+        blocks_clq = grid_blocks(cus, occ, req_waves);
+    }
+    const int64_t nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
     const int32_t cap = std::max(1, g->second_deg);
     const int32_t cap_vi = std::max(1, g->max_deg);
-    const int chunk_mult = (t && t->chunk_edges > 0) ? std::max(1, t->chunk_edges / kWave) : ((ng > 1 || np > 1) ? 4 : 1);
+    const int task_max = (t && t->chunk_edges > 0) ? t->chunk_edges : kWave;
     const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
     const int stats_on = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
@@ -1593,22 +1665,21 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     a.ctr = dc.ctr;
     a.stats = stats_on;
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
-    a.chunk_mult = chunk_mult;
+    a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
+    a.task_max = task_max;
     // Sampling is absolute: the counted roots are the edges e of the range
     // with e % root_stride == 0, so splitting a range never changes the sample.
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
     a.root_stride = root_stride;
     a.nroots = a.root_begin < a.root_end ? (a.root_end - a.root_begin + root_stride - 1) / root_stride : 0;
-    const int64_t chunks = (a.nroots + (int64_t)kWave * chunk_mult - 1) / ((int64_t)kWave * chunk_mult);
     if (np_pat > 0) {
         a.plans = d_plans;
         a.np = np_pat;
         a.sums = d_sums;
-        a.nchunks = chunks * np_pat;
         a.root_min_outdeg = 0;
         HIP_TRY(hipEventRecord(dc.evpool[0], s));
-        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, s, a);
+        enum_launch<GM_MODE_PATTERN>(nmax_pat, blocks_pat, s, a);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(dc.evpool[1], s));
         ev_list.push_back(0);
@@ -1656,13 +1727,12 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         a.plans = d_plans + np_pat + c;
         a.np = 1;
         a.sums = cell;
-        a.nchunks = chunks;
         if (ng > 1) {
             // cells for graph g at cell[g * np_clq]: use a private region instead
             a.sums = d_sums + (int64_t)ng * np_pat + (int64_t)c * ng;
         }
         HIP_TRY(hipEventRecord(dc.evpool[evl], s));
-        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_CLIQUE>, dim3(blocks), dim3(kBlock), 0, s, a);
+        enum_launch<GM_MODE_CLIQUE>(nmax_clq, blocks_clq, s, a);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(dc.evpool[evl + 1], s));
         ev_list.push_back(evl);
@@ -1772,13 +1842,13 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         gm_batch_destroy(b);
         return fail(code, msg);
     };
+    int nmax = 0;
+    for (int i = 0; i < np; ++i) nmax = std::max(nmax, nmax_for(plans[i]->host.n));
     int cus = 0, occ = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, enumerate_kernel<GM_MODE_PATTERN>, kBlock, 0) != hipSuccess)
+        enum_occupancy<GM_MODE_PATTERN>(nmax, &occ) != hipSuccess)
         return bail(GM_ERR_HIP, "gm_batch_create: device query failed");
-    occ = std::max(1, occ);
-    const int waves_per_cu = (t && t->waves_per_cu > 0) ? t->waves_per_cu : occ * kWavesPerBlock;
-    const int blocks = std::max(1, cus * std::min(waves_per_cu, occ * kWavesPerBlock) / kWavesPerBlock);
+    const int blocks = grid_blocks(cus, occ, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0);
     b->nwaves = (int64_t)blocks * kWavesPerBlock;
     const int32_t cap = std::max(1, g->second_deg), cap_vi = std::max(1, g->max_deg);
     int64_t stride = 1;
@@ -1810,7 +1880,8 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.ng = ng;
     a.sums = (unsigned long long*)b->d_sums;
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
-    a.chunk_mult = (t && t->chunk_edges > 0) ? std::max(1, t->chunk_edges / kWave) : 4;
+    a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
+    a.task_max = (t && t->chunk_edges > 0) ? t->chunk_edges : kWave;
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
@@ -1822,7 +1893,6 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.stats = b->stats;
     a.root_stride = root_stride;
     a.nroots = a.root_begin < a.root_end ? (a.root_end - a.root_begin + root_stride - 1) / root_stride : 0;
-    a.nchunks = ((a.nroots + (int64_t)kWave * a.chunk_mult - 1) / ((int64_t)kWave * a.chunk_mult)) * np;
     // Capture: zero the cells and counters with kernels (no memset nodes),
     // then the fused launch.
     hipStream_t cap_stream = nullptr;
@@ -1833,7 +1903,7 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, cap_stream, (unsigned long long*)b->d_sums, ncells);
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, cap_stream, (unsigned long long*)b->d_ctr,
                            (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
-        hipLaunchKernelGGL(enumerate_kernel<GM_MODE_PATTERN>, dim3(blocks), dim3(kBlock), 0, cap_stream, a);
+        enum_launch<GM_MODE_PATTERN>(nmax, blocks, cap_stream, a);
         hipError_t e2 = hipStreamEndCapture(cap_stream, &b->graph);
         e = (e2 != hipSuccess) ? e2 : hipGetLastError();
     }

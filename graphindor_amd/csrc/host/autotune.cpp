@@ -55,7 +55,7 @@ TuneChoice autotune_batch(const gm_graph* g, const int64_t* edge_offsets, int32_
     const int32_t k = ng;
     std::vector<uint64_t> sums((size_t)k * np);
     const int waves[] = {8, 12, 16};
-    const int chunks[] = {64, 256, 1024};
+    const int chunks[] = {16, 64, 256};
     const int tails[] = {0, -1};
     bool have = false;
     for (int w : waves)

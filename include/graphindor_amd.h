@@ -90,12 +90,14 @@ typedef struct gm_plan_desc {
 /* Launch tuning; zero fields mean "library default". */
 typedef struct gm_tuning {
     int32_t waves_per_cu;      /* resident waves per CU of the persistent grid */
-    int32_t chunk_edges;       /* root edges claimed per atomic work-steal */
+    int32_t chunk_edges;       /* most root edges one work-steal claims (claims shrink towards 1 as the
+                                  work runs out; 0 = 64) */
     int32_t lds_set_cap;       /* max elements of a prefix set staged in LDS */
     int32_t bitmap_max_degree; /* GM_MODE_CLIQUE: roots of out-degree up to this use the
                                   local-bitmap engine (0 = default 1024, -1 = off) */
-    int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted lane-parallel
-                                  (0 = default on, -1 = off: every depth wave-wide) */
+    int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted one candidate per lane
+                                  when its intersections walk at most this many elements, wave-wide
+                                  otherwise (0 = default 64, -1 = off: every depth wave-wide) */
     int32_t root_stride;       /* GM_MODE_PATTERN: count only the root edges e with e % root_stride == 0
                                   (a uniform sample for timing, identical however the range is split;
                                   0 or 1 = every edge) */
