@@ -29,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of clique_bitmap_kernel<5> on the default workload
+# (R-MAT 2^20 ef16 seed 2, degree-oriented), from two rocprofv3 PMC runs
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 1 GiB device copy
+# in the same runs); see profiles/clique5_rmat20_hbm_traffic.md.
+TRAFFIC_BITMAP5_RMAT20 = 36.12e9
 
 
 def log(*a):
@@ -270,7 +275,9 @@ def main():
                    "time_4clique_id_orientation_s": round(wall_i, 6),
                    "er_control": control},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_BITMAP5_RMAT20 if (args.scale == 20 and dom["kernel"] == "clique_bitmap_kernel<5>")
+                     else None,
                      "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 3), "alg_bytes": dom["bytes"],
                      "region_ms": round(r5["kernel_ms"], 3),
                      "other_kernel_ms": round(r5["kernel_ms"] - dom["ms"], 3)},

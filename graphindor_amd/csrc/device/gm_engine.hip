@@ -96,6 +96,20 @@ struct Counters {
     unsigned long long tasks;
     unsigned int overflow;
     unsigned int pad;
+    unsigned long long qhead;      // slice queue: next slot to pop
+    unsigned long long qtail;      // slice queue: slots reserved by pushers
+    unsigned long long pending;    // slices pushed and not yet finished
+};
+
+// A slice of one root edge's depth-2 loop set, donated by the wave that
+// found the loop set long: candidates [s0, s1) of root edge e under plan pl
+// of graph g.  `ready` is set last by the pusher and cleared by the popper,
+// so the queue is all-clear between launches.
+struct SliceItem {
+    int64_t e;
+    int32_t pl, g, s0, s1;
+    int32_t ready;
+    int32_t pad;
 };
 
 struct KernelArgs {
@@ -122,6 +136,10 @@ struct KernelArgs {
     Counters* ctr;
     int32_t stats;
     int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
+    SliceItem* queue;         // slice queue (null: no donation)
+    int64_t queue_cap;
+    int32_t slice_len;        // depth-2 candidates per donated slice
+    int32_t pad2;
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -189,6 +207,10 @@ struct Wave {
     unsigned long long lane_sum;   // per lane (lane-parallel paths)
     unsigned long long lane_bytes; // per lane
     unsigned int overflow;
+    int32_t slice_lo, slice_hi;    // depth-2 candidates this wave walks for the current root
+    int32_t donate;                // 1: long depth-2 loop sets are split and donated
+    int32_t cur_pl, cur_gr;        // plan and graph of the current root (for donated slices)
+    int64_t cur_e;                 // current root edge
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -719,6 +741,38 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     wave_sync();
 }
 
+// Pushes slices [lo + slice_len, hi) of the current root's depth-2 loop
+// set, slice_len candidates each, and returns the end of the part this wave
+// keeps.  Slots past the queue's capacity are kept too.
+__device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi) {
+    const int32_t T = w.a.slice_len;
+    const int32_t keep_hi = lo + T;
+    const int32_t nslices = (hi - keep_hi + T - 1) / T;
+    unsigned long long base = 0;
+    if (lane_id() == 0) base = atomicAdd(&w.a.ctr->qtail, (unsigned long long)nslices);
+    base = (unsigned long long)uni64(__shfl((long long)base, 0));
+    const int64_t cap = w.a.queue_cap;
+    const int32_t queued = base >= (unsigned long long)cap ? 0
+                           : (int32_t)(((int64_t)base + nslices <= cap) ? nslices : cap - (int64_t)base);
+    if (queued > 0 && lane_id() == 0) atomicAdd(&w.a.ctr->pending, (unsigned long long)queued);
+    // The last `queued` slices go to the queue; the ones before them that
+    // did not fit stay with this wave, so its part remains contiguous.
+    const int32_t first = nslices - queued;
+    for (int32_t q = lane_id(); q < queued; q += kWave) {
+        const int32_t k = first + q;
+        SliceItem* it = w.a.queue + base + q;
+        it->e = w.cur_e;
+        it->pl = w.cur_pl;
+        it->g = w.cur_gr;
+        it->s0 = keep_hi + k * T;
+        it->s1 = keep_hi + (k + 1) * T < hi ? keep_hi + (k + 1) * T : hi;
+        __threadfence();
+        __hip_atomic_store(&it->ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const int32_t kept = keep_hi + first * T;
+    return kept < hi ? kept : hi;
+}
+
 template <int D, int MODE, int NMAX>
 __device__ __forceinline__ void dfs(Wave& w) {
     if constexpr (D >= NMAX) {
@@ -753,16 +807,25 @@ __device__ __forceinline__ void dfs(Wave& w) {
         }
         const uint32_t bound = (MODE == GM_MODE_PATTERN) ? restriction_bound<D>(w) : 0xffffffffu;
         const int ln = lane_id();
-        for (int32_t base = 0; base < loop.len; base += kWave) {
+        int32_t lo = 0, hi = loop.len;
+        if constexpr (D == 2) {
+            // A root's depth-2 candidates may be split: this wave walks
+            // [slice_lo, slice_hi); a long set is cut into slices that other
+            // waves take from the queue.
+            lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
+            hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
+            if (w.donate && hi - lo > 2 * w.a.slice_len) hi = donate_slices(w, lo, hi);
+        }
+        for (int32_t base = lo; base < hi; base += kWave) {
             int32_t i = base + ln;
             uint32_t cand = 0xffffffffu;
             int64_t rb = 0, re = 0;
-            if (i < loop.len) {
+            if (i < hi) {
                 cand = loop.ptr[i];
                 rb = w.a.row[cand];
                 re = w.a.row[cand + 1];
             }
-            int32_t cnum = loop.len - base < kWave ? loop.len - base : kWave;
+            int32_t cnum = hi - base < kWave ? hi - base : kWave;
             for (int j = 0; j < cnum; ++j) {
                 uint32_t v = rdlane(cand, j);
                 if (v >= bound) return;  // sorted: nothing further qualifies
@@ -798,18 +861,52 @@ __device__ __forceinline__ int32_t edge_source(const int64_t* row, int32_t nv, i
 }
 
 template <int MODE, int NMAX>
-__device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1) {
+// Root edge (v0, v1): build the prefixes of depths 0 and 1 and walk the
+// depth-2 candidates [lo, hi).  A donated slice (`donated`) rebuilds the
+// root's prefixes without charging them again: the wave that found the root
+// charged them.
+__device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, int32_t lo, int32_t hi,
+                                             bool donated) {
     const DevPlan& p = *w.pl;
+    const int32_t stats = w.a.stats;
+    if (donated) w.a.stats = 0;
     int64_t l0 = w.a.row[v0], r0 = w.a.row[v0 + 1];
     for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0));
     int64_t l1 = w.a.row[v1], r1 = w.a.row[v1 + 1];
     for (int pid = p.last[1]; pid != -1; pid = p.next[pid]) {
         build(w, pid, w.a.col + l1, (int32_t)(r1 - l1));
-        if (get_set(w, pid).len == 0 && pid < p.basic) return;
+        if (get_set(w, pid).len == 0 && pid < p.basic) {
+            w.a.stats = stats;
+            return;
+        }
     }
+    w.a.stats = stats;
     w.emb[0] = v0;
     w.emb[1] = v1;
+    w.slice_lo = lo;
+    w.slice_hi = hi;
+    w.donate = (!donated && w.a.queue != nullptr) ? 1 : 0;
     dfs<2, MODE, NMAX>(w);
+}
+
+// Claims the next donated slice (lane 0 CAS on the queue head); -1 if none.
+__device__ __forceinline__ int64_t pop_slice(const KernelArgs& a) {
+    long long got = -1;
+    if (lane_id() == 0) {
+        unsigned long long h = __hip_atomic_load(&a.ctr->qhead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (true) {
+            const unsigned long long t = __hip_atomic_load(&a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long lim = t < (unsigned long long)a.queue_cap ? t : (unsigned long long)a.queue_cap;
+            if (h >= lim) break;
+            const unsigned long long prev = atomicCAS(&a.ctr->qhead, h, h + 1);
+            if (prev == h) {
+                got = (long long)h;
+                break;
+            }
+            h = prev;
+        }
+    }
+    return uni64(__shfl(got, 0));
 }
 
 // Sum of a per-lane value over the wave (all lanes receive it).
@@ -883,7 +980,45 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     const int64_t nwaves_total = (int64_t)gridDim.x * kWavesPerBlock;
     unsigned long long roots = 0;
     int32_t cur_g = 0, cur_p = 0;
+    bool main_done = false;
     while (true) {
+        // Donated slices first: they belong to roots already under way.
+        if (a.queue) {
+            const int64_t slot = pop_slice(a);
+            if (slot >= 0) {
+                SliceItem* it = a.queue + slot;
+                if (ln == 0)
+                    while (__hip_atomic_load(&it->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                        __builtin_amdgcn_s_sleep(1);
+                wave_sync();
+                const int64_t e = uni64(__shfl((long long)__hip_atomic_load(&it->e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0));
+                const int32_t pj = unii(__shfl(it->pl, 0));
+                const int32_t gj = unii(__shfl(it->g, 0));
+                const int32_t s0 = unii(__shfl(it->s0, 0));
+                const int32_t s1 = unii(__shfl(it->s1, 0));
+                if (ln == 0) __hip_atomic_store(&it->ready, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (gj != cur_g || pj != cur_p) {
+                    flush_sum(w, cur_g, cur_p);
+                    cur_g = gj;
+                    cur_p = pj;
+                }
+                w.pl = a.plans[pj];
+                w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
+                const uint32_t u1 = a.col[e];
+                const uint32_t u0 = (uint32_t)edge_source(a.row, a.nv, e);
+                process_root<MODE, NMAX>(w, u0, u1, s0, s1, true);
+                if (ln == 0) atomicAdd(&a.ctr->pending, ~0ull);  // one slice done
+                continue;
+            }
+        }
+        if (main_done) {
+            // Nothing to claim; stop once no donated slice is outstanding.
+            const unsigned long long pend =
+                a.queue ? __hip_atomic_load(&a.ctr->pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+            if (uni64((int64_t)__shfl((long long)pend, 0)) == 0) break;
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
         unsigned long long start = 0;
         int64_t take = 0;
         if (ln == 0) {
@@ -895,17 +1030,22 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
         }
         start = (unsigned long long)uni64((int64_t)__shfl((long long)start, 0));
         take = uni64(__shfl((long long)take, 0));
-        if ((int64_t)start >= nitems) break;
+        if ((int64_t)start >= nitems) {
+            main_done = true;
+            continue;
+        }
         const int64_t i_end = (int64_t)start + take < nitems ? (int64_t)start + take : nitems;
         for (int64_t i0 = (int64_t)start; i0 < i_end; i0 += kWave) {
             const int64_t item = i0 + ln;
             bool ok = item < i_end;
             uint32_t v0 = 0, v1 = 0;
             int32_t g = 0, pl = 0;
+            int64_t ecur = 0;
             if (ok) {
                 const int64_t ri = a.np == 1 ? item : item / a.np;
                 pl = a.np == 1 ? 0 : (int32_t)(item - ri * a.np);
                 const int64_t e = a.root_begin + ri * a.root_stride;
+                ecur = e;
                 v1 = a.col[e];
                 v0 = (uint32_t)edge_source(a.row, a.nv, e);
                 if (a.ng > 1) g = edge_graph(a.gedge, a.ng, e);
@@ -929,7 +1069,10 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
                 ++roots;
                 if (a.stats) w.bytes += 24ull;
-                process_root<MODE, NMAX>(w, u0, u1);
+                w.cur_e = rdlane64(ecur, j);
+                w.cur_pl = pj;
+                w.cur_gr = gj;
+                process_root<MODE, NMAX>(w, u0, u1, 0, 0x7fffffff, false);
             }
         }
     }
@@ -1255,7 +1398,24 @@ struct DeviceCache {
     Counters* ctr = nullptr;  // [0] call totals, [1] one clique launch, [2] one bitmap launch, [3] bitmap totals
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> evpool;  // start/stop pairs around individual launches
+    SliceItem* queue = nullptr;      // donated depth-2 slices (ready flags all-clear between launches)
 };
+
+constexpr int64_t kQueueCap = 1 << 18;
+constexpr int32_t kSliceLen = 32;
+
+// Slice length from the tuning: 0 = default, -1 = no donation.
+int32_t slice_len_of(const gm_tuning* t) {
+    if (t && t->slice_len < 0) return 0;
+    return (t && t->slice_len > 0) ? t->slice_len : kSliceLen;
+}
+
+hipError_t alloc_queue(SliceItem** q) {
+    if (*q) return hipSuccess;
+    hipError_t e = hipMalloc((void**)q, sizeof(SliceItem) * kQueueCap);
+    if (e != hipSuccess) return e;
+    return hipMemset(*q, 0, sizeof(SliceItem) * kQueueCap);
+}
 
 DeviceCache& cache_for(int dev) {
     static std::mutex mu;
@@ -1667,6 +1827,15 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
     a.task_max = task_max;
+    {
+        const int32_t sl = slice_len_of(t);
+        if (sl > 0) {
+            HIP_TRY(alloc_queue(&dc.queue));
+            a.queue = dc.queue;
+            a.queue_cap = kQueueCap;
+            a.slice_len = sl;
+        }
+    }
     // Sampling is absolute: the counted roots are the edges e of the range
     // with e % root_stride == 0, so splitting a range never changes the sample.
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
@@ -1795,6 +1964,7 @@ struct gm_batch {
     void* d_sums = nullptr;
     void* d_ctr = nullptr;
     void* d_scratch = nullptr;
+    SliceItem* d_queue = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1807,7 +1977,7 @@ void gm_batch_destroy(gm_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    for (void* q : {b->d_plans, b->d_gedge, b->d_sums, b->d_ctr, b->d_scratch})
+    for (void* q : {b->d_plans, b->d_gedge, b->d_sums, b->d_ctr, b->d_scratch, (void*)b->d_queue})
         if (q) (void)hipFree(q);
     delete b;
 }
@@ -1882,6 +2052,12 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
     a.task_max = (t && t->chunk_edges > 0) ? t->chunk_edges : kWave;
+    if (slice_len_of(t) > 0) {
+        if (alloc_queue(&b->d_queue) != hipSuccess) return bail(GM_ERR_OOM, "gm_batch_create: queue allocation failed");
+        a.queue = b->d_queue;
+        a.queue_cap = kQueueCap;
+        a.slice_len = slice_len_of(t);
+    }
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
