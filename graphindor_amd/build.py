@@ -44,10 +44,33 @@ def _run(cmd, verbose):
 
 
 def build_device(verbose=False):
+    """Compiles the device objects in parallel -- gm_engine.hip, gm_mlp.hip and
+    one object per enumerate_kernel<MODE, NMAX> instantiation of
+    enum_instance.hip -- and links libgm_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
+    obj_dir = os.path.join(LIB, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    dev = os.path.join(CSRC, "device")
+    jobs = []
+    for src in sorted(glob.glob(os.path.join(dev, "*.hip"))):
+        base = os.path.splitext(os.path.basename(src))[0]
+        if base == "enum_instance":
+            for mode in (0, 1):
+                for nmax in (4, 5, 6, 7):
+                    jobs.append((src, os.path.join(obj_dir, f"{base}_{mode}_{nmax}.o"),
+                                 [f"-DGMK_MODE={mode}", f"-DGMK_NMAX={nmax}"]))
+        else:
+            jobs.append((src, os.path.join(obj_dir, base + ".o"), []))
+    def compile_one(job):
+        src, obj, defs = job
+        _run([HIPCC, *HIP_FLAGS, *defs, f"-I{INC}", f"-I{dev}", "-c", src, "-o", obj], verbose)
+        return obj
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(compile_one, jobs))
     out = os.path.join(LIB, "libgm_hip.so")
-    srcs = sorted(glob.glob(os.path.join(CSRC, "device", "*.hip")))
-    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INC}", *srcs, "-o", out + ".tmp"], verbose)
+    _run([HIPCC, *HIP_FLAGS, "-shared", *objs, "-o", out + ".tmp"], verbose)
     os.replace(out + ".tmp", out)
     return out
 

@@ -29,1061 +29,11 @@
 
 #include "graphindor_amd.h"
 
+#include "engine_kernels.h"
+
 namespace {
 
-constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWave * kWavesPerBlock;
-constexpr int kMaxSets = GM_MAX_PREFIX + GM_MAX_PATTERN;  // prefixes + per-depth filtered loop sets
-
-// Plan as the device reads it (wave-uniform scalar loads).
-struct DevPlan {
-    int32_t n, P, basic, nrestrict, k, vinduced, mode, iep_vn, iep_an;
-    int32_t adj[GM_MAX_PATTERN * GM_MAX_PATTERN];
-    int32_t father[GM_MAX_PREFIX], last[GM_MAX_PATTERN], next[GM_MAX_PREFIX], brk[GM_MAX_PREFIX];
-    int32_t loopset[GM_MAX_PATTERN], rlast[GM_MAX_PATTERN], rnext[GM_MAX_PREFIX], rindex[GM_MAX_PREFIX];
-    int32_t size_only[GM_MAX_PREFIX];
-    int32_t clique_tail;  // clique mode: depth n-2 counts lane-parallel (see clique_tail_count)
-    int32_t clique_chain; // clique mode on the plain k-clique chain: local-bitmap engine eligible
-    int32_t tail_depth;   // pattern mode: depth counted lane-parallel by pattern_tail (-1: none)
-    int32_t iep_id[GM_MAX_IEP], iep_vflag[GM_MAX_IEP], iep_vcoef[GM_MAX_IEP];
-    int32_t iep_coef[GM_MAX_IEP], iep_flag[GM_MAX_IEP], iep_pos[GM_MAX_IEP];
-};
-
-struct SetRef {
-    const uint32_t* ptr;
-    int32_t len;
-    int32_t pad;
-};
-
-// Elements of a clique-mode tail loop set staged in LDS per wave.
-constexpr int kStage = 2048;
-
-// Pattern-rule last level (pattern_tail): prefixes and IEP rows it handles.
-constexpr int kTailMaxPrefix = 4;
-constexpr int kTailMaxIep = 8;
-
-// Everything the last level reads that does not depend on the candidate.
-struct TailCtx {
-    int32_t pid[kTailMaxPrefix];
-    const uint32_t* fptr[kTailMaxPrefix];
-    int32_t flen[kTailMaxPrefix];
-    bool alias[kTailMaxPrefix];
-    int32_t base[kTailMaxIep];
-    const uint32_t* sptr[kTailMaxIep];
-    int32_t slen[kTailMaxIep];
-    int32_t at[kTailMaxIep];  // position of the IEP row's prefix in pid[], or -1
-    int qat;                  // position of the last loop set in pid[] (no IEP), or -1
-    SetRef qset;              // the last loop set when it was built above D
-};
-
-struct WaveLds {
-    SetRef sets[kMaxSets];
-    int32_t ans[GM_MAX_IEP];
-    TailCtx tail;  // pattern_tail's candidate-independent state (LDS, not registers)
-};
-
-template <int MODE>
-struct WaveShared {
-    WaveLds w;
-    uint32_t stage[kStage];
-};
-
-struct Counters {
-    unsigned long long bm_cursor;  // local-bitmap engine's root cursor
-    unsigned long long cursor;     // enumerate kernel's task cursor
-    unsigned long long bytes;
-    unsigned long long tasks;
-    unsigned int overflow;
-    unsigned int pad;
-    unsigned long long qhead;      // slice queue: next slot to pop
-    unsigned long long qtail;      // slice queue: slots reserved by pushers
-    unsigned long long pending;    // slices pushed and not yet finished
-};
-
-// A slice of one root edge's depth-2 loop set, donated by the wave that
-// found the loop set long: candidates [s0, s1) of root edge e under plan pl
-// of graph g.  `ready` is set last by the pusher and cleared by the popper,
-// so the queue is all-clear between launches.
-struct SliceItem {
-    int64_t e;
-    int32_t pl, g, s0, s1;
-    int32_t ready;
-    int32_t pad;
-};
-
-struct KernelArgs {
-    const int64_t* row;
-    const uint32_t* col;
-    int32_t nv;
-    int64_t nnz;
-    const DevPlan* const* plans;  // np plans counted in one launch
-    int32_t np;
-    const int64_t* gedge;         // ng+1 root-edge offsets: graph g owns edges [gedge[g], gedge[g+1])
-    int32_t ng;
-    unsigned long long* sums;     // [ng * np] raw sums
-    int32_t tail_off;             // 1: never use the lane-parallel pattern tail
-    int32_t tail_work;            // pattern tail: candidates walking more elements than this go wave-wide
-    int32_t task_max;             // most work items one claim takes
-    int64_t root_begin, root_end; // root edges counted: [gedge[0], gedge[ng])
-    int64_t root_stride;          // counted root edges: root_begin + i * root_stride < root_end (root_begin is
-                                  // the range's first multiple of root_stride)
-    int64_t nroots;               // number of counted root edges
-    uint32_t* scratch;
-    int64_t wave_stride;  // uint32 per wave
-    int32_t cap;          // capacity of a prefix slot
-    int32_t cap_vi;       // capacity of a vertex-induced slot
-    Counters* ctr;
-    int32_t stats;
-    int64_t root_min_outdeg;  // clique mode: skip roots v0 whose |N(v0)| <= this (taken by the bitmap engine)
-    SliceItem* queue;         // slice queue (null: no donation)
-    int64_t queue_cap;
-    int32_t slice_len;        // depth-2 candidates per donated slice
-    int32_t pad2;
-};
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ int32_t unii(int32_t x) { return (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)x); }
-__device__ __forceinline__ int64_t uni64(int64_t x) {
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ uint32_t rdlane(uint32_t x, int l) { return __builtin_amdgcn_readlane(x, l); }
-__device__ __forceinline__ int64_t rdlane64(int64_t x, int l) {
-    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(uint64_t)x, l);
-    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)x >> 32), l);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-__device__ __forceinline__ int below(uint64_t m) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Branchless search for `key` in sorted b[0:nb): the window halves with a
-// trip count that depends only on nb.
-__device__ __forceinline__ bool contains(const uint32_t* b, int32_t nb, uint32_t key) {
-    if (nb <= 0) return false;
-    const uint32_t* base = b;
-    int32_t n = nb;
-    while (n > 1) {
-        int32_t half = n >> 1;
-        base = (base[half] <= key) ? base + half : base;
-        n -= half;
-    }
-    return *base == key;
-}
-
-// First index i with b[i] >= key (unsigned), single lane.
-__device__ __forceinline__ int32_t lower_bound_u(const uint32_t* b, int32_t nb, uint32_t key) {
-    int32_t lo = 0, len = nb;
-    while (len > 0) {
-        int32_t half = len >> 1;
-        if (b[lo + half] < key) {
-            lo += half + 1;
-            len -= half + 1;
-        } else {
-            len = half;
-        }
-    }
-    return lo;
-}
-
-struct Wave {
-    KernelArgs a;
-    const DevPlan* pl;
-    int tail_depth;
-    WaveLds* lds;
-    uint32_t* stage;
-    uint32_t* scratch;
-    uint32_t emb[GM_MAX_PATTERN + 1];
-    unsigned long long sum;        // wave-uniform
-    unsigned long long bytes;      // wave-uniform
-    unsigned long long lane_sum;   // per lane (lane-parallel paths)
-    unsigned long long lane_bytes; // per lane
-    unsigned int overflow;
-    int32_t slice_lo, slice_hi;    // depth-2 candidates this wave walks for the current root
-    int32_t donate;                // 1: long depth-2 loop sets are split and donated
-    int32_t cur_pl, cur_gr;        // plan and graph of the current root (for donated slices)
-    int64_t cur_e;                 // current root edge
-};
-
-__device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
-    SetRef s = w.lds->sets[pid];
-    SetRef r;
-    r.ptr = (const uint32_t*)uni64((int64_t)s.ptr);
-    r.len = unii(s.len);
-    r.pad = 0;
-    return r;
-}
-
-__device__ __forceinline__ void put_set(Wave& w, int pid, const uint32_t* ptr, int32_t len) {
-    if (lane_id() == 0) {
-        w.lds->sets[pid].ptr = ptr;
-        w.lds->sets[pid].len = len;
-    }
-    wave_sync();
-}
-
-template <int D>
-__device__ __forceinline__ uint32_t emb_at(const Wave& w, int idx) {
-    uint32_t r = 0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) r = (idx == j) ? w.emb[j] : r;
-    return r;
-}
-
-template <int D>
-__device__ __forceinline__ bool in_emb(const Wave& w, uint32_t v) {
-    bool hit = false;
-#pragma unroll
-    for (int j = 0; j < D; ++j) hit |= (w.emb[j] == v);
-    return hit;
-}
-
-// Wave-cooperative |a ∩ b|, compacted into out (out == nullptr: size only).
-__device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* out,
-                             int32_t cap) {
-    if (na > nb) {
-        const uint32_t* t = a;
-        a = b;
-        b = t;
-        int32_t tn = na;
-        na = nb;
-        nb = tn;
-    }
-    int32_t cnt = 0;
-    if (na == 0) return 0;
-    const int ln = lane_id();
-    for (int32_t base = 0; base < na; base += kWave) {
-        int32_t i = base + ln;
-        bool hit = false;
-        uint32_t x = 0;
-        if (i < na) {
-            x = a[i];
-            hit = contains(b, nb, x);
-        }
-        uint64_t m = ballot(hit);
-        if (out != nullptr && hit) {
-            int32_t pos = cnt + below(m);
-            if (pos < cap) out[pos] = x;
-        }
-        cnt += (int32_t)__popcll(m);
-    }
-    if (out != nullptr) {
-        if (cnt > cap) w.overflow = 1;
-        wave_sync();
-    }
-    return cnt;
-}
-
-// |s[0:lim) \ emb[0:D)|
-template <int D>
-__device__ __forceinline__ int32_t subtract_emb(const Wave& w, const uint32_t* s, int32_t lim) {
-    const int ln = lane_id();
-    uint32_t key = emb_at<D>(w, ln);
-    bool hit = (ln < D) && contains(s, lim, key);
-    return lim - (int32_t)__popcll(ballot(hit));
-}
-
-__device__ __forceinline__ void build(Wave& w, int pid, const uint32_t* in, int32_t nin) {
-    const DevPlan& p = *w.pl;
-    int f = p.father[pid];
-    if (f < 0) {
-        put_set(w, pid, in, nin);
-        return;
-    }
-    SetRef fs = get_set(w, f);
-    if (p.size_only[pid]) {
-        int32_t c = intersect(w, fs.ptr, fs.len, in, nin, nullptr, 0);
-        put_set(w, pid, nullptr, c);
-        if (w.a.stats) w.bytes += 4ull * (unsigned long long)(fs.len + nin);
-        return;
-    }
-    uint32_t* out = w.scratch + (int64_t)pid * w.a.cap;
-    int32_t c = intersect(w, fs.ptr, fs.len, in, nin, out, w.a.cap);
-    put_set(w, pid, out, c < w.a.cap ? c : w.a.cap);  // overflow is reported, never read past
-    if (w.a.stats) w.bytes += 4ull * (unsigned long long)(fs.len + nin + c);
-}
-
-template <int D>
-__device__ __forceinline__ uint32_t restriction_bound(const Wave& w) {
-    const DevPlan& p = *w.pl;
-    uint32_t b = 0xffffffffu;
-    for (int i = p.rlast[D]; i != -1; i = p.rnext[i]) {
-        uint32_t e = emb_at<D>(w, p.rindex[i]);
-        b = e < b ? e : b;
-    }
-    return b;
-}
-
-// Vertex-induced filter (gpu/function/pattern_matching.cuh:145-206): keep the
-// loop-set elements adjacent to no embedded vertex the pattern marks as a
-// non-neighbour of vertex D.
-template <int D>
-__device__ __forceinline__ SetRef remove_anti_edge(Wave& w, SetRef in) {
-    const DevPlan& p = *w.pl;
-    uint32_t* out = w.scratch + (int64_t)p.P * w.a.cap + (int64_t)D * w.a.cap_vi;
-    const int ln = lane_id();
-    int32_t cnt = 0;
-    for (int32_t base = 0; base < in.len; base += kWave) {
-        int32_t i = base + ln;
-        bool keep = false;
-        uint32_t x = 0;
-        if (i < in.len) {
-            x = in.ptr[i];
-            keep = true;
-#pragma unroll
-            for (int u = 0; u < D; ++u) {
-                if (p.adj[u * p.n + D] != 0) continue;
-                uint32_t v = w.emb[u];
-                int64_t l = w.a.row[v], r = w.a.row[v + 1];
-                if (contains(w.a.col + l, (int32_t)(r - l), x)) keep = false;
-            }
-        }
-        uint64_t m = ballot(keep);
-        if (keep) {
-            int32_t pos = cnt + below(m);
-            if (pos < w.a.cap_vi) out[pos] = x;
-        }
-        cnt += (int32_t)__popcll(m);
-    }
-    if (cnt > w.a.cap_vi) w.overflow = 1;
-    wave_sync();
-    SetRef r;
-    r.ptr = out;
-    r.len = cnt < w.a.cap_vi ? cnt : w.a.cap_vi;
-    r.pad = 0;
-    return r;
-}
-
-// IEP terminal (gpu/function/pattern_matching.cuh:110-139).
-template <int D>
-__device__ __forceinline__ void iep_terminal(Wave& w) {
-    const DevPlan& p = *w.pl;
-    for (int i = 0; i < p.iep_vn; ++i) {
-        SetRef s = get_set(w, p.iep_id[i]);
-        int32_t v;
-        if (p.iep_vflag[i]) {
-            v = s.len - p.iep_vcoef[i];
-        } else {
-            v = subtract_emb<D>(w, s.ptr, s.len);
-            if (w.a.stats) w.bytes += 4ull * D;
-        }
-        if (lane_id() == 0) w.lds->ans[i] = v;
-    }
-    wave_sync();
-    int last_pos = -1;
-    long long val = 0;
-    for (int pos = 0; pos < p.iep_an; ++pos) {
-        long long x = unii(w.lds->ans[p.iep_pos[pos]]);
-        if (pos == last_pos + 1) val = x;
-        else if (val != 0) val = (long long)((unsigned long long)val * (unsigned long long)x);
-        if (p.iep_flag[pos]) {
-            last_pos = pos;
-            w.sum += (unsigned long long)val * (unsigned long long)(long long)p.iep_coef[pos];
-        }
-    }
-}
-
-// Branchless membership test in an LDS-resident sorted array.
-__device__ __forceinline__ bool lds_contains(const uint32_t* b, int32_t nb, uint32_t key) {
-    if (nb <= 0) return false;
-    int32_t base = 0, n = nb;
-    while (n > 1) {
-        int32_t half = n >> 1;
-        base = (b[base + half] <= key) ? base + half : base;
-        n -= half;
-    }
-    return b[base] == key;
-}
-
-// Clique rule at depth n-2: the count is sum over v in L of |L ∩ N(v)|, the
-// size of the last loop set each candidate would build
-// (gpu/gpu_clique.cu:224-250).  L is staged in LDS and every lane takes its
-// own candidates, iterating the shorter of L and N(v) and searching the
-// other.  Per candidate the byte model charges what the wave path does:
-// 12 B (element + row pointers) + 4 (|L| + |N(v)|) for a size-only
-// intersection.
-__device__ __forceinline__ void clique_tail_count(Wave& w, SetRef L) {
-    const int ln = lane_id();
-    for (int32_t i = ln; i < L.len; i += kWave) w.stage[i] = L.ptr[i];
-    wave_sync();
-    unsigned long long acc = 0, bytes = 0;
-    for (int32_t i = ln; i < L.len; i += kWave) {
-        uint32_t v = w.stage[i];
-        int64_t l = w.a.row[v], r = w.a.row[v + 1];
-        int32_t d = (int32_t)(r - l);
-        const uint32_t* nb = w.a.col + l;
-        int32_t c = 0;
-        if (d <= L.len) {
-            for (int32_t j = 0; j < d; ++j) c += lds_contains(w.stage, L.len, nb[j]) ? 1 : 0;
-        } else {
-            for (int32_t j = 0; j < L.len; ++j) c += contains(nb, d, w.stage[j]) ? 1 : 0;
-        }
-        acc += (unsigned long long)c;
-        bytes += 12ull + 4ull * (unsigned long long)(L.len + d);
-    }
-    w.lane_sum += acc;
-    if (w.a.stats) w.lane_bytes += bytes;
-    wave_sync();
-}
-
-
-// ---------------------------------------------------------------------------
-// Last candidate level of the pattern rule, lane-parallel where it pays.
-//
-// At depth D = n-k-1 (IEP tail of k >= 2 vertices) or D = n-2 (no IEP) the
-// sets the schedule builds for a candidate v are only read for their sizes
-// and for membership of the embedded vertices: every prefix q of last[D]
-// is S_q = F_q ∩ N(v) with F_q (its father) fixed above D, or N(v) itself.
-// So a candidate's contribution is |S_q| for each q, the break-size prune,
-// and then either the IEP terminal of depth D+1
-// (gpu/function/pattern_matching.cuh:110-139) or the restricted last level
-// (pattern_matching.cuh:240-257), with nothing materialised.
-//
-// Light candidates -- whose intersections walk at most kLaneTailWork
-// elements in all -- are evaluated one per lane, each lane iterating the
-// shorter list and searching the longer.  Heavy ones (hub neighbourhoods)
-// would hold the whole wave up while one lane walks thousands of elements,
-// so they are evaluated one at a time by the whole wave, 64 elements per
-// step.  Both evaluate the same arithmetic; the fathers are staged in LDS
-// when they fit.  Byte model: as the wave path, except that the builds of
-// depth D are charged as size-only.
-// ---------------------------------------------------------------------------
-constexpr int32_t kLaneTailWork = 64;  // default light/heavy bound (gm_tuning.lane_tail > 0 sets it)
-
-// |F ∩ N| over the elements below `limit`, iterating the shorter list (per lane).
-__device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
-                                                    uint32_t limit) {
-    int32_t c = 0;
-    if (nF <= nN) {
-        for (int32_t t = 0; t < nF; ++t) {
-            uint32_t x = F[t];
-            if (x >= limit) break;
-            c += contains(Nv, nN, x) ? 1 : 0;
-        }
-    } else {
-        for (int32_t t = 0; t < nN; ++t) {
-            uint32_t x = Nv[t];
-            if (x >= limit) break;
-            c += contains(F, nF, x) ? 1 : 0;
-        }
-    }
-    return c;
-}
-
-// The same count by the whole wave (uniform arguments and result): lane i
-// takes element i of each 64-element step of the shorter list.
-__device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
-                                                    uint32_t limit) {
-    const uint32_t* s = F;
-    const uint32_t* l = Nv;
-    int32_t ns = nF, nl = nN;
-    if (nF > nN) {
-        s = Nv;
-        l = F;
-        ns = nN;
-        nl = nF;
-    }
-    const int32_t lim = (limit == 0xffffffffu) ? ns : lower_bound_u(s, ns, limit);
-    int32_t c = 0;
-    for (int32_t b = 0; b < lim; b += kWave) {
-        const int32_t i = b + lane_id();
-        const bool hit = i < lim && contains(l, nl, s[i]);
-        c += (int32_t)__popcll(ballot(hit));
-    }
-    return c;
-}
-
-
-// Contribution of candidate v (not embedded, below the bound) to the sum,
-// and its byte charge beyond the 12 bytes of reading v and its row.  WAVE:
-// every lane holds the same v and the counts are wave-wide; otherwise each
-// lane evaluates its own v.
-template <int D, bool WAVE>
-__device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, uint32_t v, const uint32_t* Nv,
-                                               int32_t dv, unsigned long long& acc, unsigned long long& bytes) {
-    const DevPlan& p = *w.pl;
-    auto count = [&](const uint32_t* F, int32_t nF, const uint32_t* N2, int32_t n2, uint32_t limit) -> int32_t {
-        if constexpr (WAVE) return wave_isect_count(F, nF, N2, n2, limit);
-        else return lane_isect_count(F, nF, N2, n2, limit);
-    };
-    int32_t sz[kTailMaxPrefix];
-#pragma unroll
-    for (int j = 0; j < kTailMaxPrefix; ++j) {
-        sz[j] = 0;
-        if (c.pid[j] == -1) continue;
-        if (c.alias[j]) {
-            sz[j] = dv;
-        } else {
-            sz[j] = count(c.fptr[j], c.flen[j], Nv, dv, 0xffffffffu);
-            bytes += 4ull * (unsigned long long)(c.flen[j] + dv);
-        }
-        if (sz[j] == p.brk[c.pid[j]]) return;
-    }
-    // x in S_j  <=>  x in N(v) and (alias or x in F_j).
-    auto member = [&](int j, uint32_t x) -> bool {
-        bool in_n = contains(Nv, dv, x);
-        bool in_f = true;
-#pragma unroll
-        for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-            if (jj == j && !c.alias[jj]) in_f = contains(c.fptr[jj], c.flen[jj], x);
-        return in_n && in_f;
-    };
-    if (p.k >= 2) {
-        int32_t ans[kTailMaxIep];
-#pragma unroll
-        for (int e = 0; e < kTailMaxIep; ++e) {
-            ans[e] = 0;
-            if (e >= p.iep_vn) continue;
-            const int j = c.at[e];
-            if (j >= 0) {
-                int32_t szj = 0;
-#pragma unroll
-                for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-                    if (jj == j) szj = sz[jj];
-                if (p.iep_vflag[e]) {
-                    ans[e] = szj - p.iep_vcoef[e];
-                } else {
-                    int32_t cnt = szj;
-#pragma unroll
-                    for (int x = 0; x < D; ++x) cnt -= member(j, w.emb[x]) ? 1 : 0;
-                    ans[e] = cnt;
-                    bytes += 4ull * (D + 1);
-                }
-            } else if (p.iep_vflag[e]) {
-                ans[e] = c.base[e];
-            } else {
-                ans[e] = c.base[e] - (contains(c.sptr[e], c.slen[e], v) ? 1 : 0);
-                bytes += 4ull * (D + 1);
-            }
-        }
-        int last_pos = -1;
-        long long val = 0;
-        for (int pos = 0; pos < p.iep_an; ++pos) {
-            const int ap = p.iep_pos[pos];
-            long long x = 0;
-#pragma unroll
-            for (int e = 0; e < kTailMaxIep; ++e)
-                if (e == ap) x = ans[e];
-            if (pos == last_pos + 1) val = x;
-            else if (val != 0) val = (long long)((unsigned long long)val * (unsigned long long)x);
-            if (p.iep_flag[pos]) {
-                last_pos = pos;
-                acc += (unsigned long long)val * (unsigned long long)(long long)p.iep_coef[pos];
-            }
-        }
-    } else {
-        // Depth D+1 = n-1: |{x in S : x < bound'} \ emb'| with emb' = emb + v.
-        uint32_t b1 = 0xffffffffu;
-        for (int rr = p.rlast[D + 1]; rr != -1; rr = p.rnext[rr]) {
-            const int idx = p.rindex[rr];
-            uint32_t ev = (idx == D) ? v : emb_at<D>(w, idx);
-            b1 = ev < b1 ? ev : b1;
-        }
-        int32_t res;
-        if (c.qat >= 0) {
-            bool al = false;
-            const uint32_t* F = nullptr;
-            int32_t nF = 0;
-#pragma unroll
-            for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-                if (jj == c.qat) {
-                    al = c.alias[jj];
-                    F = c.fptr[jj];
-                    nF = c.flen[jj];
-                }
-            res = al ? lower_bound_u(Nv, dv, b1) : count(F, nF, Nv, dv, b1);
-#pragma unroll
-            for (int x = 0; x < D; ++x)
-                if (w.emb[x] < b1 && member(c.qat, w.emb[x])) --res;
-        } else {
-            const int32_t lim = lower_bound_u(c.qset.ptr, c.qset.len, b1);
-            res = lim;
-#pragma unroll
-            for (int x = 0; x < D; ++x)
-                if (contains(c.qset.ptr, lim, w.emb[x])) --res;
-            if (contains(c.qset.ptr, lim, v)) --res;
-        }
-        acc += (unsigned long long)(long long)res;
-        bytes += 4ull * (D + 1);
-    }
-}
-
-template <int D>
-__device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound);
-
-template <int D>
-__device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
-    const DevPlan& p = *w.pl;
-    const int ln = lane_id();
-    const uint32_t bound = restriction_bound<D>(w);
-    TailCtx c;  // built here, then kept in LDS for the candidate loop
-    // Prefixes of depth D and their fathers (staged in LDS when they fit).
-    int q = p.last[D];
-    int32_t staged = 0;
-#pragma unroll
-    for (int j = 0; j < kTailMaxPrefix; ++j) {
-        c.pid[j] = q;
-        c.alias[j] = true;
-        c.fptr[j] = nullptr;
-        c.flen[j] = 0;
-        if (q != -1) {
-            int f = p.father[q];
-            if (f >= 0) {
-                SetRef fs = get_set(w, f);
-                c.alias[j] = false;
-                c.fptr[j] = fs.ptr;
-                c.flen[j] = fs.len;
-                staged += fs.len;
-            }
-            q = p.next[q];
-        }
-    }
-    if (staged <= kStage) {
-        int32_t off = 0;
-#pragma unroll
-        for (int j = 0; j < kTailMaxPrefix; ++j) {
-            if (c.pid[j] == -1 || c.alias[j]) continue;
-            for (int32_t t = ln; t < c.flen[j]; t += kWave) w.stage[off + t] = c.fptr[j][t];
-            c.fptr[j] = w.stage + off;
-            off += c.flen[j];
-        }
-        wave_sync();
-    }
-    // IEP rows on sets built above D: |S| - coef, or |S \ emb[0:D)| (uniform).
-#pragma unroll
-    for (int e = 0; e < kTailMaxIep; ++e) {
-        c.base[e] = 0;
-        c.sptr[e] = nullptr;
-        c.slen[e] = 0;
-        c.at[e] = -1;
-        if (p.k >= 2 && e < p.iep_vn) {
-            int ip = p.iep_id[e];
-#pragma unroll
-            for (int j = 0; j < kTailMaxPrefix; ++j)
-                if (c.pid[j] == ip) c.at[e] = j;
-            if (c.at[e] < 0) {
-                SetRef s = get_set(w, ip);
-                c.sptr[e] = s.ptr;
-                c.slen[e] = s.len;
-                c.base[e] = p.iep_vflag[e] ? s.len - p.iep_vcoef[e] : subtract_emb<D>(w, s.ptr, s.len);
-            }
-        }
-    }
-    // Last loop set when there is no IEP tail.
-    c.qat = -1;
-    c.qset.ptr = nullptr;
-    c.qset.len = 0;
-    c.qset.pad = 0;
-    if (p.k == 0) {
-        const int qlast = p.loopset[D + 1];
-#pragma unroll
-        for (int j = 0; j < kTailMaxPrefix; ++j)
-            if (c.pid[j] == qlast) c.qat = j;
-        if (c.qat < 0) c.qset = get_set(w, qlast);
-    }
-    if (ln == 0) w.lds->tail = c;
-    wave_sync();
-    return pattern_tail_loop<D>(w, loop, bound);
-}
-
-template <int D>
-__device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound) {
-    const int ln = lane_id();
-    const TailCtx& c = w.lds->tail;
-    unsigned long long acc = 0, bytes = 0;
-    for (int32_t b0 = 0; b0 < loop.len; b0 += kWave) {
-        const int32_t i = b0 + ln;
-        const bool in_range = i < loop.len;
-        const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
-        const bool live = in_range && v < bound;
-        const bool stop_after = ballot(in_range && v >= bound) != 0;
-        bool heavy = false;
-        if (live) {
-            bytes += 4ull;
-            if (!in_emb<D>(w, v)) {
-                bytes += 8ull;
-                const int64_t l = w.a.row[v], r = w.a.row[v + 1];
-                const int32_t dv = (int32_t)(r - l);
-                int32_t work = 0;
-#pragma unroll
-                for (int j = 0; j < kTailMaxPrefix; ++j)
-                    if (c.pid[j] != -1 && !c.alias[j]) work += c.flen[j] < dv ? c.flen[j] : dv;
-                heavy = work > w.a.tail_work;
-                if (!heavy) tail_candidate<D, false>(w, c, v, w.a.col + l, dv, acc, bytes);
-            }
-        }
-        // Heavy candidates, one at a time, wave-wide.
-        uint64_t hm = ballot(heavy);
-        while (hm != 0) {
-            const int j = __ffsll((unsigned long long)hm) - 1;
-            hm &= hm - 1;
-            const uint32_t hv = rdlane(v, j);
-            const int64_t l = w.a.row[hv], r = w.a.row[hv + 1];
-            unsigned long long a2 = 0, b2 = 0;
-            tail_candidate<D, true>(w, c, hv, w.a.col + l, (int32_t)(r - l), a2, b2);
-            if (ln == 0) {
-                acc += a2;
-                bytes += b2;
-            }
-        }
-        if (stop_after) break;
-    }
-    w.lane_sum += acc;
-    if (w.a.stats) w.lane_bytes += bytes;
-    wave_sync();
-}
-
-// Pushes slices [lo + slice_len, hi) of the current root's depth-2 loop
-// set, slice_len candidates each, and returns the end of the part this wave
-// keeps.  Slots past the queue's capacity are kept too.
-__device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi) {
-    const int32_t T = w.a.slice_len;
-    const int32_t keep_hi = lo + T;
-    const int32_t nslices = (hi - keep_hi + T - 1) / T;
-    unsigned long long base = 0;
-    if (lane_id() == 0) base = atomicAdd(&w.a.ctr->qtail, (unsigned long long)nslices);
-    base = (unsigned long long)uni64(__shfl((long long)base, 0));
-    const int64_t cap = w.a.queue_cap;
-    const int32_t queued = base >= (unsigned long long)cap ? 0
-                           : (int32_t)(((int64_t)base + nslices <= cap) ? nslices : cap - (int64_t)base);
-    if (queued > 0 && lane_id() == 0) atomicAdd(&w.a.ctr->pending, (unsigned long long)queued);
-    // The last `queued` slices go to the queue; the ones before them that
-    // did not fit stay with this wave, so its part remains contiguous.
-    const int32_t first = nslices - queued;
-    for (int32_t q = lane_id(); q < queued; q += kWave) {
-        const int32_t k = first + q;
-        SliceItem* it = w.a.queue + base + q;
-        it->e = w.cur_e;
-        it->pl = w.cur_pl;
-        it->g = w.cur_gr;
-        it->s0 = keep_hi + k * T;
-        it->s1 = keep_hi + (k + 1) * T < hi ? keep_hi + (k + 1) * T : hi;
-        __threadfence();
-        __hip_atomic_store(&it->ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const int32_t kept = keep_hi + first * T;
-    return kept < hi ? kept : hi;
-}
-
-template <int D, int MODE, int NMAX>
-__device__ __forceinline__ void dfs(Wave& w) {
-    if constexpr (D >= NMAX) {
-        return;
-    } else {
-        const DevPlan& p = *w.pl;
-        const int n = p.n;
-        if (MODE == GM_MODE_PATTERN && D == n - p.k) {
-            iep_terminal<D>(w);
-            return;
-        }
-        SetRef loop = get_set(w, p.loopset[D]);
-        if (MODE == GM_MODE_PATTERN && p.vinduced) loop = remove_anti_edge<D>(w, loop);
-        if (D == n - 1) {
-            if (MODE == GM_MODE_CLIQUE) {
-                w.sum += (unsigned long long)loop.len;
-                return;
-            }
-            uint32_t bound = restriction_bound<D>(w);
-            int32_t lim = unii(lower_bound_u(loop.ptr, loop.len, bound));
-            w.sum += (unsigned long long)(long long)subtract_emb<D>(w, loop.ptr, lim);
-            if (w.a.stats) w.bytes += 4ull * D;
-            return;
-        }
-        if (MODE == GM_MODE_CLIQUE && D == n - 2 && p.clique_tail && loop.len <= kStage) {
-            clique_tail_count(w, loop);
-            return;
-        }
-        if (MODE == GM_MODE_PATTERN && D == w.tail_depth) {
-            pattern_tail<D>(w, loop);
-            return;
-        }
-        const uint32_t bound = (MODE == GM_MODE_PATTERN) ? restriction_bound<D>(w) : 0xffffffffu;
-        const int ln = lane_id();
-        int32_t lo = 0, hi = loop.len;
-        if constexpr (D == 2) {
-            // A root's depth-2 candidates may be split: this wave walks
-            // [slice_lo, slice_hi); a long set is cut into slices that other
-            // waves take from the queue.
-            lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
-            hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
-            if (w.donate && hi - lo > 2 * w.a.slice_len) hi = donate_slices(w, lo, hi);
-        }
-        for (int32_t base = lo; base < hi; base += kWave) {
-            int32_t i = base + ln;
-            uint32_t cand = 0xffffffffu;
-            int64_t rb = 0, re = 0;
-            if (i < hi) {
-                cand = loop.ptr[i];
-                rb = w.a.row[cand];
-                re = w.a.row[cand + 1];
-            }
-            int32_t cnum = hi - base < kWave ? hi - base : kWave;
-            for (int j = 0; j < cnum; ++j) {
-                uint32_t v = rdlane(cand, j);
-                if (v >= bound) return;  // sorted: nothing further qualifies
-                if (w.a.stats) w.bytes += 4ull;
-                if (MODE == GM_MODE_PATTERN && in_emb<D>(w, v)) continue;
-                int64_t l = rdlane64(rb, j), r = rdlane64(re, j);
-                if (w.a.stats) w.bytes += 8ull;
-                bool prune = false;
-                for (int pid = p.last[D]; pid != -1; pid = p.next[pid]) {
-                    build(w, pid, w.a.col + l, (int32_t)(r - l));
-                    if (get_set(w, pid).len == p.brk[pid]) {
-                        prune = true;
-                        break;
-                    }
-                }
-                if (prune) continue;
-                w.emb[D] = v;
-                dfs<D + 1, MODE, NMAX>(w);
-            }
-        }
-    }
-}
-
-// Largest v with row[v] <= e (uniform trip count).
-__device__ __forceinline__ int32_t edge_source(const int64_t* row, int32_t nv, int64_t e) {
-    int32_t base = 0, n = nv;  // search in row[0:nv)
-    while (n > 1) {
-        int32_t half = n >> 1;
-        base = (row[base + half] <= e) ? base + half : base;
-        n -= half;
-    }
-    return base;
-}
-
-template <int MODE, int NMAX>
-// Root edge (v0, v1): build the prefixes of depths 0 and 1 and walk the
-// depth-2 candidates [lo, hi).  A donated slice (`donated`) rebuilds the
-// root's prefixes without charging them again: the wave that found the root
-// charged them.
-__device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, int32_t lo, int32_t hi,
-                                             bool donated) {
-    const DevPlan& p = *w.pl;
-    const int32_t stats = w.a.stats;
-    if (donated) w.a.stats = 0;
-    int64_t l0 = w.a.row[v0], r0 = w.a.row[v0 + 1];
-    for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0));
-    int64_t l1 = w.a.row[v1], r1 = w.a.row[v1 + 1];
-    for (int pid = p.last[1]; pid != -1; pid = p.next[pid]) {
-        build(w, pid, w.a.col + l1, (int32_t)(r1 - l1));
-        if (get_set(w, pid).len == 0 && pid < p.basic) {
-            w.a.stats = stats;
-            return;
-        }
-    }
-    w.a.stats = stats;
-    w.emb[0] = v0;
-    w.emb[1] = v1;
-    w.slice_lo = lo;
-    w.slice_hi = hi;
-    w.donate = (!donated && w.a.queue != nullptr) ? 1 : 0;
-    dfs<2, MODE, NMAX>(w);
-}
-
-// Claims the next donated slice (lane 0 CAS on the queue head); -1 if none.
-__device__ __forceinline__ int64_t pop_slice(const KernelArgs& a) {
-    long long got = -1;
-    if (lane_id() == 0) {
-        unsigned long long h = __hip_atomic_load(&a.ctr->qhead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (true) {
-            const unsigned long long t = __hip_atomic_load(&a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned long long lim = t < (unsigned long long)a.queue_cap ? t : (unsigned long long)a.queue_cap;
-            if (h >= lim) break;
-            const unsigned long long prev = atomicCAS(&a.ctr->qhead, h, h + 1);
-            if (prev == h) {
-                got = (long long)h;
-                break;
-            }
-            h = prev;
-        }
-    }
-    return uni64(__shfl(got, 0));
-}
-
-// Sum of a per-lane value over the wave (all lanes receive it).
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-        lo = __shfl_xor(lo, off);
-        hi = __shfl_xor(hi, off);
-        x += ((unsigned long long)hi << 32) | lo;
-    }
-    return x;
-}
-
-// Graph of root edge e: largest g with gedge[g] <= e (uniform trip count).
-__device__ __forceinline__ int32_t edge_graph(const int64_t* gedge, int32_t ng, int64_t e) {
-    int32_t base = 0, n = ng;
-    while (n > 1) {
-        int32_t half = n >> 1;
-        base = (gedge[base + half] <= e) ? base + half : base;
-        n -= half;
-    }
-    return base;
-}
-
-// Adds the wave's partial sum for (graph, plan) to sums[] and clears it.
-__device__ __forceinline__ void flush_sum(Wave& w, int32_t g, int32_t pi) {
-    const unsigned long long total = w.sum + wave_sum(w.lane_sum);
-    if (lane_id() == 0 && total != 0ull) atomicAdd(&w.a.sums[(int64_t)g * w.a.np + pi], total);
-    w.sum = 0;
-    w.lane_sum = 0;
-}
-
-// Persistent enumeration over tasks (root chunk c, plan pi), chunk-major so
-// the plans of one chunk run close together and share its neighbour lists
-// in cache.  A root's count goes to the (graph, plan) cell of its edge.
-// NMAX: the largest pattern size among the launch's plans; the DFS is
-// instantiated only that deep, which keeps registers (and so occupancy) to
-// what the patterns need.
-template <int MODE, int NMAX>
-__global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
-    __shared__ WaveShared<MODE> lds_all[kWavesPerBlock];
-    Wave w;
-    w.a = a;
-    const int wid = threadIdx.x / kWave;
-    w.lds = &lds_all[wid].w;
-    w.stage = lds_all[wid].stage;
-    const int64_t gwave = (int64_t)blockIdx.x * kWavesPerBlock + wid;
-    w.scratch = a.scratch + gwave * a.wave_stride;
-    w.sum = 0;
-    w.bytes = 0;
-    w.lane_sum = 0;
-    w.lane_bytes = 0;
-    w.overflow = 0;
-#pragma unroll
-    for (int i = 0; i <= GM_MAX_PATTERN; ++i) w.emb[i] = 0;
-    // A set that is never built reads as empty.
-    for (int i = lane_id(); i < kMaxSets; i += kWave) {
-        w.lds->sets[i].ptr = nullptr;
-        w.lds->sets[i].len = 0;
-    }
-    wave_sync();
-    const int ln = lane_id();
-    // Work items are (logical root, plan) pairs, root-major, so the waves
-    // that share a root edge's neighbour lists run its plans close together.
-    // Guided self-scheduling: a claim takes remaining / (4 * waves) items,
-    // at most task_max and at least 1, so large claims amortise the cursor
-    // atomic early and single roots spread over the whole grid at the end
-    // (a hub's root edges can each hold seconds of work).
-    const int64_t nitems = a.nroots * (int64_t)a.np;
-    const int64_t nwaves_total = (int64_t)gridDim.x * kWavesPerBlock;
-    unsigned long long roots = 0;
-    int32_t cur_g = 0, cur_p = 0;
-    bool main_done = false;
-    while (true) {
-        // Donated slices first: they belong to roots already under way.
-        if (a.queue) {
-            const int64_t slot = pop_slice(a);
-            if (slot >= 0) {
-                SliceItem* it = a.queue + slot;
-                if (ln == 0)
-                    while (__hip_atomic_load(&it->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
-                        __builtin_amdgcn_s_sleep(1);
-                wave_sync();
-                const int64_t e = uni64(__shfl((long long)__hip_atomic_load(&it->e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0));
-                const int32_t pj = unii(__shfl(it->pl, 0));
-                const int32_t gj = unii(__shfl(it->g, 0));
-                const int32_t s0 = unii(__shfl(it->s0, 0));
-                const int32_t s1 = unii(__shfl(it->s1, 0));
-                if (ln == 0) __hip_atomic_store(&it->ready, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (gj != cur_g || pj != cur_p) {
-                    flush_sum(w, cur_g, cur_p);
-                    cur_g = gj;
-                    cur_p = pj;
-                }
-                w.pl = a.plans[pj];
-                w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
-                const uint32_t u1 = a.col[e];
-                const uint32_t u0 = (uint32_t)edge_source(a.row, a.nv, e);
-                process_root<MODE, NMAX>(w, u0, u1, s0, s1, true);
-                if (ln == 0) atomicAdd(&a.ctr->pending, ~0ull);  // one slice done
-                continue;
-            }
-        }
-        if (main_done) {
-            // Nothing to claim; stop once no donated slice is outstanding.
-            const unsigned long long pend =
-                a.queue ? __hip_atomic_load(&a.ctr->pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-            if (uni64((int64_t)__shfl((long long)pend, 0)) == 0) break;
-            __builtin_amdgcn_s_sleep(8);
-            continue;
-        }
-        unsigned long long start = 0;
-        int64_t take = 0;
-        if (ln == 0) {
-            const unsigned long long seen = __hip_atomic_load(&a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int64_t rem = nitems - (int64_t)seen;
-            take = rem / (4 * nwaves_total);
-            take = take < 1 ? 1 : (take > a.task_max ? a.task_max : take);
-            start = atomicAdd(&a.ctr->cursor, (unsigned long long)take);
-        }
-        start = (unsigned long long)uni64((int64_t)__shfl((long long)start, 0));
-        take = uni64(__shfl((long long)take, 0));
-        if ((int64_t)start >= nitems) {
-            main_done = true;
-            continue;
-        }
-        const int64_t i_end = (int64_t)start + take < nitems ? (int64_t)start + take : nitems;
-        for (int64_t i0 = (int64_t)start; i0 < i_end; i0 += kWave) {
-            const int64_t item = i0 + ln;
-            bool ok = item < i_end;
-            uint32_t v0 = 0, v1 = 0;
-            int32_t g = 0, pl = 0;
-            int64_t ecur = 0;
-            if (ok) {
-                const int64_t ri = a.np == 1 ? item : item / a.np;
-                pl = a.np == 1 ? 0 : (int32_t)(item - ri * a.np);
-                const int64_t e = a.root_begin + ri * a.root_stride;
-                ecur = e;
-                v1 = a.col[e];
-                v0 = (uint32_t)edge_source(a.row, a.nv, e);
-                if (a.ng > 1) g = edge_graph(a.gedge, a.ng, e);
-                if (a.plans[pl]->rlast[1] != -1 && v0 <= v1) ok = false;
-                if (MODE == GM_MODE_CLIQUE && a.root_min_outdeg > 0 && a.row[v0 + 1] - a.row[v0] <= a.root_min_outdeg)
-                    ok = false;
-            }
-            uint64_t m = ballot(ok);
-            while (m != 0) {
-                int j = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                const uint32_t u0 = rdlane(v0, j), u1 = rdlane(v1, j);
-                const int32_t gj = (int32_t)rdlane((uint32_t)g, j);
-                const int32_t pj = (int32_t)rdlane((uint32_t)pl, j);
-                if (gj != cur_g || pj != cur_p) {
-                    flush_sum(w, cur_g, cur_p);
-                    cur_g = gj;
-                    cur_p = pj;
-                }
-                w.pl = a.plans[pj];
-                w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
-                ++roots;
-                if (a.stats) w.bytes += 24ull;
-                w.cur_e = rdlane64(ecur, j);
-                w.cur_pl = pj;
-                w.cur_gr = gj;
-                process_root<MODE, NMAX>(w, u0, u1, 0, 0x7fffffff, false);
-            }
-        }
-    }
-    flush_sum(w, cur_g, cur_p);
-    const unsigned long long lane_bytes = a.stats ? wave_sum(w.lane_bytes) : 0ull;
-    if (ln == 0) {
-        atomicAdd(&a.ctr->tasks, roots);
-        if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes + lane_bytes);
-        if (w.overflow) atomicOr(&a.ctr->overflow, 1u);
-    }
-}
+using namespace gmk;
 
 // Adds a clique launch's counters into the call's totals.
 __global__ void accumulate_counters_kernel(Counters* total, const Counters* part) {
@@ -1098,239 +48,6 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
         p[i] = 0ull;
 }
 
-
-// ---------------------------------------------------------------------------
-// Local-bitmap clique engine (GM_MODE_CLIQUE, plain k-clique chain).
-//
-// For a root v0 of out-degree d <= kBmMaxD every later candidate set is a
-// subset of N(v0), so the enumeration below v0 runs on the d-vertex local
-// DAG: row i is the bitmap (over positions in N(v0)) of N(N[i]) ∩ N(v0),
-// built once per root.  Then a k-clique through v0 is a chain i -> j -> ...
-// and every set operation of the schedule is a W-word AND (W = ceil(d/64)):
-//   |S_{v0,i,j}|            = popcount(row_i & row_j)
-//   |S_{v0,i,j} ∩ N(l)|     = popcount(row_i & row_j & row_l), l in row_i & row_j
-// The (i, j) edges of the local DAG are spread evenly over the lanes through
-// an exclusive scan of the row popcounts.  This is the local-subgraph bitmap
-// idea of the reference's gpu/gpu_kclique.cu:359-400, applied to the
-// schedule's clique chain; counts equal the list engine's.
-//
-// Byte model (HBM, per root): 16 (row pointers) + 4 d (N(v0)) +
-// sum over u in N(v0) of (8 + 4 |N(u)|) for building the rows; the counting
-// phase reads only the rows (on chip).
-// ---------------------------------------------------------------------------
-constexpr int kBmMaxD = 1024;      // largest root out-degree handled with bitmaps
-constexpr int kBmMaxW = kBmMaxD / 64;
-constexpr int kBmLdsWords = 1024;  // rows kept in LDS when d * W <= this
-
-struct BmArgs {
-    const int64_t* row;
-    const uint32_t* col;
-    int32_t nv;
-    int32_t k;                 // clique size
-    int32_t max_d;             // roots with out-degree above this are left to the list engine
-    unsigned long long* rows_scratch;  // per wave kBmMaxD * kBmMaxW words
-    int64_t nchunks;           // root chunks of kBmChunk vertices
-    unsigned long long* out;   // the plan's sum cell
-    Counters* ctr;
-    int32_t stats;
-};
-constexpr int kBmChunk = 16;
-
-struct BmShared {
-    uint32_t nbr[kBmMaxD];
-    int32_t off[kBmMaxD + 1];
-    unsigned long long rows[kBmLdsWords];
-};
-
-// Position of the r-th set bit (0-based) of x; x has more than r set bits.
-__device__ __forceinline__ int select_bit64(unsigned long long x, int r) {
-    int pos = 0;
-#pragma unroll
-    for (int width = 32; width >= 1; width >>= 1) {
-        unsigned long long low = x & ((width == 64 ? ~0ull : ((1ull << width) - 1)));
-        int c = __popcll(low);
-        if (r >= c) {
-            r -= c;
-            x >>= width;
-            pos += width;
-        }
-    }
-    return pos;
-}
-
-template <int WMAX, int K>
-__device__ unsigned long long bm_count_pair(const unsigned long long* rows, int W, int i, int j) {
-    unsigned long long S[WMAX];
-#pragma unroll
-    for (int w = 0; w < WMAX; ++w) S[w] = (w < W) ? (rows[i * W + w] & rows[j * W + w]) : 0ull;
-    unsigned long long acc = 0;
-    if constexpr (K == 4) {
-#pragma unroll
-        for (int w = 0; w < WMAX; ++w) acc += __popcll(S[w]);
-    } else {
-#pragma unroll
-        for (int w = 0; w < WMAX; ++w) {
-            unsigned long long m = S[w];
-            while (m) {
-                int l = w * 64 + __builtin_ctzll(m);
-                m &= m - 1;
-                const unsigned long long* rl = rows + (int64_t)l * W;
-                if constexpr (K == 5) {
-#pragma unroll
-                    for (int x = 0; x < WMAX; ++x)
-                        if (x < W) acc += __popcll(S[x] & rl[x]);
-                } else {
-                    unsigned long long T[WMAX];
-#pragma unroll
-                    for (int x = 0; x < WMAX; ++x) T[x] = (x < W) ? (S[x] & rl[x]) : 0ull;
-#pragma unroll
-                    for (int x = 0; x < WMAX; ++x) {
-                        unsigned long long mm = T[x];
-                        while (mm) {
-                            int q = x * 64 + __builtin_ctzll(mm);
-                            mm &= mm - 1;
-                            const unsigned long long* rq = rows + (int64_t)q * W;
-                            if constexpr (K == 6) {
-#pragma unroll
-                                for (int y = 0; y < WMAX; ++y)
-                                    if (y < W) acc += __popcll(T[y] & rq[y]);
-                            } else {  // K == 7
-#pragma unroll
-                                for (int y = 0; y < WMAX; ++y) {
-                                    if (y >= W) continue;
-                                    unsigned long long mq = T[y] & rq[y];
-                                    while (mq) {
-                                        int z = y * 64 + __builtin_ctzll(mq);
-                                        mq &= mq - 1;
-                                        const unsigned long long* rz = rows + (int64_t)z * W;
-#pragma unroll
-                                        for (int t = 0; t < WMAX; ++t)
-                                            if (t < W) acc += __popcll(T[t] & rq[t] & rz[t]);
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
-    return acc;
-}
-
-template <int WMAX, int K>
-__device__ unsigned long long bm_root(const BmArgs& a, BmShared& sh, unsigned long long* gl_rows, uint32_t v0,
-                                      int64_t l0, int32_t d, unsigned long long& bytes) {
-    const int ln = lane_id();
-    const int W = (d + 63) >> 6;
-    for (int i = ln; i < d; i += kWave) sh.nbr[i] = a.col[l0 + i];
-    unsigned long long* rows = (d * W <= kBmLdsWords) ? sh.rows : gl_rows;
-    for (int t = ln; t < d * W; t += kWave) rows[t] = 0ull;
-    wave_sync();
-    // Row i = positions in N(v0) of N(N[i]); lane-private rows, no conflicts.
-    unsigned long long my_bytes = 0;
-    for (int i = ln; i < d; i += kWave) {
-        uint32_t u = sh.nbr[i];
-        int64_t lu = a.row[u], ru = a.row[u + 1];
-        int32_t du = (int32_t)(ru - lu);
-        my_bytes += 8ull + 4ull * (unsigned long long)du;
-        for (int32_t t = 0; t < du; ++t) {
-            uint32_t x = a.col[lu + t];
-            int32_t base = 0, n = d;
-            while (n > 1) {
-                int32_t half = n >> 1;
-                base = (sh.nbr[base + half] <= x) ? base + half : base;
-                n -= half;
-            }
-            if (sh.nbr[base] == x) rows[i * W + (base >> 6)] |= 1ull << (base & 63);
-        }
-    }
-    bytes += my_bytes;
-    wave_sync();
-    unsigned long long acc = 0;
-    if constexpr (K == 3) {
-        for (int t = ln; t < d * W; t += kWave) acc += __popcll(rows[t]);
-        return acc;
-    } else {
-        // Exclusive scan of row popcounts -> off[0..d].
-        int32_t carry = 0;
-        for (int base = 0; base < d; base += kWave) {
-            int i = base + ln;
-            int32_t c = 0;
-            if (i < d)
-                for (int w = 0; w < W; ++w) c += __popcll(rows[i * W + w]);
-            int32_t incl = c;
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                int32_t y = __shfl_up(incl, off);
-                if (ln >= off) incl += y;
-            }
-            if (i < d) sh.off[i] = carry + incl - c;
-            carry += __shfl(incl, kWave - 1);
-        }
-        if (ln == 0) sh.off[d] = carry;
-        wave_sync();
-        const int32_t P = carry;
-        for (int32_t pidx = ln; pidx < P; pidx += kWave) {
-            // Row i with off[i] <= pidx < off[i+1].
-            int32_t lo = 0, n = d;
-            while (n > 1) {
-                int32_t half = n >> 1;
-                lo = (sh.off[lo + half] <= pidx) ? lo + half : lo;
-                n -= half;
-            }
-            const int i = lo;
-            int r = pidx - sh.off[i];
-            int j = 0;
-            for (int w = 0; w < W; ++w) {
-                unsigned long long word = rows[i * W + w];
-                int c = __popcll(word);
-                if (r < c) {
-                    j = w * 64 + select_bit64(word, r);
-                    break;
-                }
-                r -= c;
-            }
-            acc += bm_count_pair<WMAX, K>(rows, W, i, j);
-        }
-        wave_sync();
-        return acc;
-    }
-}
-
-template <int WMAX, int K>
-__global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
-    __shared__ BmShared sh;
-    const int ln = lane_id();
-    unsigned long long* gl_rows = a.rows_scratch + (int64_t)blockIdx.x * kBmMaxD * kBmMaxW;
-    unsigned long long acc = 0, bytes = 0, wave_bytes = 0;
-    unsigned long long roots = 0;
-    while (true) {
-        unsigned long long chunk = 0;
-        if (ln == 0) chunk = atomicAdd(&a.ctr->bm_cursor, 1ull);
-        chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
-        if ((long long)chunk >= a.nchunks) break;
-        const int32_t vbeg = (int32_t)(chunk * kBmChunk);
-        const int32_t vend = vbeg + kBmChunk < a.nv ? vbeg + kBmChunk : a.nv;
-        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
-            int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
-            int32_t d = unii((int32_t)(r0 - l0));
-            if (d < K - 1 || d > a.max_d) continue;
-            ++roots;
-            if (a.stats) wave_bytes += 16ull + 4ull * (unsigned long long)d;
-            if (d <= 64) acc += bm_root<1, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
-            else if (d <= 256) acc += bm_root<4, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
-            else acc += bm_root<WMAX, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
-        }
-    }
-    const unsigned long long total = wave_sum(acc);
-    const unsigned long long tb = a.stats ? wave_sum(bytes) : 0ull;
-    if (ln == 0) {
-        atomicAdd(a.out, total);
-        atomicAdd(&a.ctr->tasks, roots);
-        if (a.stats) atomicAdd(&a.ctr->bytes, tb + wave_bytes);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Host side of the C ABI.
@@ -1359,20 +76,20 @@ int nmax_for(int n) { return n <= 4 ? 4 : n; }
 template <int MODE>
 hipError_t enum_occupancy(int nmax, int* blocks_per_cu) {
     switch (nmax) {
-        case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 4>, kBlock, 0);
-        case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 5>, kBlock, 0);
-        case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 6>, kBlock, 0);
-        default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, enumerate_kernel<MODE, 7>, kBlock, 0);
+        case 4: return MODE == 0 ? enum_occupancy_0_4(blocks_per_cu) : enum_occupancy_1_4(blocks_per_cu);
+        case 5: return MODE == 0 ? enum_occupancy_0_5(blocks_per_cu) : enum_occupancy_1_5(blocks_per_cu);
+        case 6: return MODE == 0 ? enum_occupancy_0_6(blocks_per_cu) : enum_occupancy_1_6(blocks_per_cu);
+        default: return MODE == 0 ? enum_occupancy_0_7(blocks_per_cu) : enum_occupancy_1_7(blocks_per_cu);
     }
 }
 
 template <int MODE>
 void enum_launch(int nmax, int blocks, hipStream_t s, const KernelArgs& a) {
     switch (nmax) {
-        case 4: hipLaunchKernelGGL((enumerate_kernel<MODE, 4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((enumerate_kernel<MODE, 5>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((enumerate_kernel<MODE, 6>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-        default: hipLaunchKernelGGL((enumerate_kernel<MODE, 7>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+        case 4: MODE == 0 ? enum_launch_0_4(blocks, s, a) : enum_launch_1_4(blocks, s, a); break;
+        case 5: MODE == 0 ? enum_launch_0_5(blocks, s, a) : enum_launch_1_5(blocks, s, a); break;
+        case 6: MODE == 0 ? enum_launch_0_6(blocks, s, a) : enum_launch_1_6(blocks, s, a); break;
+        default: MODE == 0 ? enum_launch_0_7(blocks, s, a) : enum_launch_1_7(blocks, s, a); break;
     }
 }
 

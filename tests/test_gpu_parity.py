@@ -435,3 +435,31 @@ def test_root_stride_counts_exactly_the_multiples(stride):
     assert sum(p[0] for p in pieces) == sampled[0][0]
     full, _ = N.count(dg, dp)
     assert N.count_batch(dg, [0, nnz], [dp], root_stride=1)[0][0][0] == full
+
+
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "c6"])
+def test_slice_donation_keeps_sums_and_bytes(name):
+    """Depth-2 slices donated to idle waves (slice lengths 1, 4 and the
+    default; 1 overflows the 2^18-slot queue, so the kept-locally path runs
+    too) give the undonated sum and byte tally, equal to the oracle's."""
+    n, adj = PATTERNS[name]
+    g = N.Graph.rmat(10, 8, 17)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    s = N.Schedule.build(n, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
+    dg = N.DeviceGraph(v, e)
+    dp = N.DevicePlan(s.plan_desc())
+    off, st_off = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, slice_len=-1, lane_tail=-1)
+    oraw, ob = O.count_edge_rooted(O.CSR(v, e), s.plan_desc())
+    assert (off, st_off.alg_bytes) == (oraw, ob)
+    for sl in (1, 4, 0):
+        for tail in (-1, 0):
+            print(f"slice_len {sl} lane_tail {tail}", flush=True)
+            got, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, slice_len=sl, lane_tail=tail)
+            assert got == off and st.alg_bytes == ob, (sl, tail)
+    # captured batch with donation, replayed
+    b = N.Batch(dg, [0, len(e)], [dp], slice_len=1)
+    for _ in range(3):
+        assert b.run()[0][0][0] == off
+    b.free()
