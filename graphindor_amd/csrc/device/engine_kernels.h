@@ -796,7 +796,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
             // waves take from the queue.
             lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
             hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
-            if (w.donate && hi - lo > 2 * w.a.slice_len) hi = donate_slices(w, lo, hi);
+            if (w.donate && hi - lo > 4 * w.a.slice_len) hi = donate_slices(w, lo, hi);
         }
         for (int32_t base = lo; base < hi; base += kWave) {
             int32_t i = base + ln;

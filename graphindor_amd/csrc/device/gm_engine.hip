@@ -544,7 +544,9 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
     a.task_max = task_max;
-    {
+    // Slice donation applies to the pattern rule; clique plans rely on the
+    // degree orientation and the bitmap engine to bound a root's work.
+    if (np_pat > 0) {
         const int32_t sl = slice_len_of(t);
         if (sl > 0) {
             HIP_TRY(alloc_queue(&dc.queue));

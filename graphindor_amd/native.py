@@ -509,14 +509,12 @@ class DevicePlan:
             pass
 
 
-def count(dgraph, dplan, flags=0, waves_per_cu=0, bitmap_max_degree=0, lane_tail=0):
-    """Raw sum (before the redundancy division) and the gm_stats, via gm_count."""
+def count(dgraph, dplan, flags=0, **tuning):
+    """Raw sum (before the redundancy division) and the gm_stats, via gm_count.
+    Keyword arguments are gm_tuning fields (see _tuning)."""
     raw = ctypes.c_uint64(0)
     st = GmStats()
-    tun = GmTuning()
-    tun.waves_per_cu = waves_per_cu
-    tun.bitmap_max_degree = bitmap_max_degree
-    tun.lane_tail = lane_tail
+    tun = _tuning(**tuning)
     _gm_check(gm().gm_count(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw), ctypes.byref(st)),
               "gm_count")
     return raw.value, st

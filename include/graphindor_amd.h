@@ -101,9 +101,9 @@ typedef struct gm_tuning {
     int32_t root_stride;       /* GM_MODE_PATTERN: count only the root edges e with e % root_stride == 0
                                   (a uniform sample for timing, identical however the range is split;
                                   0 or 1 = every edge) */
-    int32_t slice_len;         /* a root whose depth-2 candidate set is longer than 2*slice_len keeps
-                                  slice_len of them and donates the rest, in slices of slice_len, to
-                                  idle waves (0 = default 32, -1 = off) */
+    int32_t slice_len;         /* GM_MODE_PATTERN: a root whose depth-2 candidate set is longer than
+                                  4*slice_len keeps slice_len of them and donates the rest, in slices
+                                  of slice_len, to idle waves (0 = default 32, -1 = off) */
     int32_t reserved[1];
 } gm_tuning;
 
