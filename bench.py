@@ -147,6 +147,59 @@ def batch_leg(N, count=512, stride=256):
             "autotune_s": round(tune_s, 3), "sums_equal": True}
 
 
+PM5 = [("house", "0101110101010101010011000"), ("c5", "0100110100010100010110010"),
+       ("k4_pendant", "0111110110110101110010000")]
+MODEL_PATH = os.path.join(ROOT, "graphindor_amd", "data", "cost_model.gscm")
+
+
+def pm_leg(N, scale=22, seed=4, budget_s=8.0):
+    """5-vertex pattern matching with IEP on R-MAT 2^scale ef16 (house, 5-cycle,
+    4-clique + pendant): the plain analytic schedule and, when a trained cost
+    model is present, the learned pick.  A full count on R-MAT 2^22 is far
+    beyond a bench run (hub neighbourhoods of 10^5 vertices), so each pattern
+    is timed on the root sample e % S == 0, S halving from 2^16 until the
+    analytic run takes about budget_s / 8; both schedules use the same S.
+    Sampled sums differ between schedules (a root edge maps to different
+    embeddings), so the comparison is of times; full-count parity is in the
+    GPU tests."""
+    t0 = time.time()
+    g = N.Graph.rmat(scale, 16, seed)
+    inf = g.info()
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    gen_s = time.time() - t0
+    dg = N.DeviceGraph(v, e)
+    model = N.CostModel.load(MODEL_PATH) if os.path.exists(MODEL_PATH) else None
+    out = {"graph": f"R-MAT 2^{scale} ef16 seed {seed}", "csr_entries": int(len(e)),
+           "max_degree": int((v[1:] - v[:-1]).max()), "model": os.path.basename(MODEL_PATH) if model else None,
+           "patterns": {}}
+    for name, adj in PM5:
+        row = {}
+        sa, ia = N.select_schedule(5, adj, v, inf["tri_cnt"], None)
+        pa = N.DevicePlan(sa.plan_desc())
+        S = 1 << 16
+        while True:
+            _, st = N.count_batch(dg, [0, len(e)], [pa], root_stride=S)
+            if st.kernel_ms / 1e3 >= budget_s / 8 or S == 1:
+                break
+            S //= 2
+        _, st = N.count_batch(dg, [0, len(e)], [pa], root_stride=S)
+        row.update({"root_stride": S, "analytic_ms": round(st.kernel_ms, 3),
+                    "analytic_candidate": ia["analytic"], "candidates": ia["candidates"],
+                    "select_ms_analytic": round(ia["select_ms"], 2)})
+        if model is not None:
+            sl, il = N.select_schedule(5, adj, v, inf["tri_cnt"], model)
+            pl = N.DevicePlan(sl.plan_desc())
+            _, st2 = N.count_batch(dg, [0, len(e)], [pl], root_stride=S)
+            row.update({"learned_ms": round(st2.kernel_ms, 3), "learned_candidate": il["chosen"],
+                        "select_ms_learned": round(il["select_ms"], 2),
+                        "learned_speedup": round(st.kernel_ms / max(st2.kernel_ms, 1e-6), 3)})
+        out["patterns"][name] = row
+        log(f"pm {name}: stride {S}: {row}")
+    out["gen_s"] = round(gen_s, 1)
+    return out
+
+
 def calibration_copy(mb):
     """One device-to-device copy of `mb` MiB (HIP's own copy kernel), so that
     a PMC run sees a kernel of known traffic next to the counting kernels."""
@@ -175,6 +228,8 @@ def main():
     ap.add_argument("--no-control", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched motif leg")
     ap.add_argument("--batch-stride", type=int, default=256)
+    ap.add_argument("--no-pm", action="store_true", help="skip the 5-vertex pattern-matching leg")
+    ap.add_argument("--pm-scale", type=int, default=22)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
@@ -284,6 +339,8 @@ def main():
     }
     if not args.no_batch:
         out["config"]["batch_4motif"] = batch_leg(N, 512, args.batch_stride)
+    if not args.no_pm:
+        out["config"]["pm_5vertex"] = pm_leg(N, args.pm_scale, 4)
     if not args.no_cpu:
         from oracle import oracle as O
         out["cpu_baseline"] = cpu_baseline(N, O, iv.copy(), ie.copy(), 5)

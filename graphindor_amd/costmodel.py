@@ -104,7 +104,7 @@ def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=
     return idx, ms, stride
 
 
-def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, log=print):
+def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, log=print, budget_s=30.0):
     """Dataset over R-MAT graphs (scale, edge_factor, seed) x patterns."""
     rng = np.random.default_rng(seed)
     xs, ts, goff, meta = [], [], [0], []
@@ -120,7 +120,7 @@ def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, 
             if len(cands) < 2:
                 continue
             x = N.encode_candidates(n, adj, v, inf["tri_cnt"])
-            idx, ms, stride = time_group(dg, len(e), n, adj, cands, target_ms, reps, max_cands, rng, log)
+            idx, ms, stride = time_group(dg, len(e), n, adj, cands, target_ms, reps, max_cands, rng, log, budget_s)
             xs.append(x[idx])
             ts.append(np.log(np.maximum(np.asarray(ms, dtype=np.float64), 1e-4)).astype(np.float32))
             goff.append(goff[-1] + len(idx))
@@ -148,8 +148,10 @@ def group_regret(model, ds):
     return out
 
 
-def evaluate(model, scale, ef, seed, patterns, log=print):
-    """Full (unsampled) counts on one R-MAT graph: analytic vs learned pick."""
+def evaluate(model, scale, ef, seed, patterns, log=print, root_stride=1):
+    """Analytic vs learned pick on one R-MAT graph.  root_stride 1: full
+    counts, which must agree; S > 1: both schedules timed on the root sample
+    e % S == 0 (sampled sums are not counts and differ between schedules)."""
     g = N.Graph.rmat(scale, ef, seed)
     v, e = g.csr()
     v, e = v.copy(), e.copy()
@@ -161,13 +163,16 @@ def evaluate(model, scale, ef, seed, patterns, log=print):
         for kind, mdl in (("analytic", None), ("learned", model)):
             s, info = N.select_schedule(n, adj, v, inf["tri_cnt"], mdl)
             dp = N.DevicePlan(s.plan_desc())
-            raw, st = N.count(dg, dp)
-            res[kind] = {"count": raw // s.redundancy, "ms": st.kernel_ms, "choice": info}
-        assert res["analytic"]["count"] == res["learned"]["count"], name
+            raw, st = N.count_batch(dg, [0, len(e)], [dp], root_stride=root_stride)
+            res[kind] = {"raw": raw[0][0], "redundancy": s.redundancy, "ms": st.kernel_ms, "choice": info}
+        if root_stride == 1:
+            ca = res["analytic"]["raw"] // res["analytic"]["redundancy"]
+            cl = res["learned"]["raw"] // res["learned"]["redundancy"]
+            assert ca == cl, name
         rows.append({"pattern": name, **res})
-        log(f"  {name}: count {res['analytic']['count']} analytic {res['analytic']['ms']:.1f} ms "
-            f"learned {res['learned']['ms']:.1f} ms (candidate {res['learned']['choice']['chosen']} vs "
-            f"{res['analytic']['choice']['analytic']} of {res['analytic']['choice']['candidates']})")
+        log(f"  {name}: analytic {res['analytic']['ms']:.1f} ms learned {res['learned']['ms']:.1f} ms "
+            f"(candidate {res['learned']['choice']['chosen']} vs {res['analytic']['choice']['analytic']} of "
+            f"{res['analytic']['choice']['candidates']}, stride {root_stride})")
     return rows
 
 
@@ -182,17 +187,19 @@ def main(argv=None):
     t.add_argument("--target-ms", type=float, default=4.0)
     t.add_argument("--max-cands", type=int, default=96)
     t.add_argument("--dataset", default=None, help="also write the timing dataset (.npz)")
+    t.add_argument("--budget-s", type=float, default=30.0, help="timing budget per (pattern, graph) group")
     e = sub.add_parser("eval")
     e.add_argument("--model", required=True)
     e.add_argument("--scale", type=int, default=20)
     e.add_argument("--ef", type=int, default=16)
     e.add_argument("--seed", type=int, default=4)
+    e.add_argument("--stride", type=int, default=1)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         t0 = time.time()
         pats = random_connected_patterns(a.patterns, seed=6)
         specs = [(int(s), 16, 6 + i) for i, s in enumerate(a.scales.split(","))]
-        ds = collect(specs, pats, a.target_ms, 2, a.max_cands)
+        ds = collect(specs, pats, a.target_ms, 2, a.max_cands, budget_s=a.budget_s)
         print(f"dataset: {len(ds['log_ms'])} timings in {len(ds['goff']) - 1} groups, {time.time() - t0:.1f} s",
               flush=True)
         m, loss = train(ds, steps=a.steps)
@@ -206,7 +213,8 @@ def main(argv=None):
         m = N.CostModel.load(a.model)
         pats = [("house", 5, "0101110101010101010011000"), ("c5", 5, "0100110100010100010110010"),
                 ("k4_pendant", 5, "0111110110110101110010000")]
-        evaluate(m, a.scale, a.ef, a.seed, pats)
+        pats += [("c6", 6, "010001101000010100001010000101100010"), ("prism", 6, "011100101010110001100011010101001110")]
+        evaluate(m, a.scale, a.ef, a.seed, pats, root_stride=a.stride)
     return 0
 
 

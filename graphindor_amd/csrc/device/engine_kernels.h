@@ -121,7 +121,7 @@ struct KernelArgs {
     SliceItem* queue;         // slice queue (null: no donation)
     int64_t queue_cap;
     int32_t slice_len;        // depth-2 candidates per donated slice
-    int32_t pad2;
+    int32_t eager_donation;   // 1: donate long remainders at once, not only when roots run out
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -193,6 +193,7 @@ struct Wave {
     int32_t donate;                // 1: long depth-2 loop sets are split and donated
     int32_t cur_pl, cur_gr;        // plan and graph of the current root (for donated slices)
     int64_t cur_e;                 // current root edge
+    int64_t nitems;                // (root, plan) work items of the launch
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -792,13 +793,21 @@ __device__ __forceinline__ void dfs(Wave& w) {
         int32_t lo = 0, hi = loop.len;
         if constexpr (D == 2) {
             // A root's depth-2 candidates may be split: this wave walks
-            // [slice_lo, slice_hi); a long set is cut into slices that other
-            // waves take from the queue.
+            // [slice_lo, slice_hi) (a donated slice, or the whole set).
             lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
             hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
-            if (w.donate && hi - lo > 4 * w.a.slice_len) hi = donate_slices(w, lo, hi);
         }
         for (int32_t base = lo; base < hi; base += kWave) {
+            if constexpr (D == 2) {
+                // Lazy donation: once no root is left to claim, other waves
+                // would idle, so a long remainder is cut into slices for them.
+                if (w.donate && hi - base > 4 * w.a.slice_len) {
+                    unsigned long long c = 0;
+                    if (ln == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    c = (unsigned long long)uni64(__shfl((long long)c, 0));
+                    if ((int64_t)c >= w.nitems || w.a.eager_donation) hi = donate_slices(w, base, hi);
+                }
+            }
             int32_t i = base + ln;
             uint32_t cand = 0xffffffffu;
             int64_t rb = 0, re = 0;
@@ -959,6 +968,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     // atomic early and single roots spread over the whole grid at the end
     // (a hub's root edges can each hold seconds of work).
     const int64_t nitems = a.nroots * (int64_t)a.np;
+    w.nitems = nitems;
     const int64_t nwaves_total = (int64_t)gridDim.x * kWavesPerBlock;
     unsigned long long roots = 0;
     int32_t cur_g = 0, cur_p = 0;

@@ -553,6 +553,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             a.queue = dc.queue;
             a.queue_cap = kQueueCap;
             a.slice_len = sl;
+            a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
         }
     }
     // Sampling is absolute: the counted roots are the edges e of the range
@@ -776,6 +777,7 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         a.queue = b->d_queue;
         a.queue_cap = kQueueCap;
         a.slice_len = slice_len_of(t);
+        a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
     }
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;

@@ -22,6 +22,7 @@ GM_ERR_UNSUPPORTED = -4
 GM_MODE_PATTERN = 0
 GM_MODE_CLIQUE = 1
 GM_COUNT_ALG_BYTES = 1
+GM_COUNT_EAGER_DONATION = 2
 GM_MAX_PATTERN = 7
 
 

@@ -101,9 +101,10 @@ typedef struct gm_tuning {
     int32_t root_stride;       /* GM_MODE_PATTERN: count only the root edges e with e % root_stride == 0
                                   (a uniform sample for timing, identical however the range is split;
                                   0 or 1 = every edge) */
-    int32_t slice_len;         /* GM_MODE_PATTERN: a root whose depth-2 candidate set is longer than
-                                  4*slice_len keeps slice_len of them and donates the rest, in slices
-                                  of slice_len, to idle waves (0 = default 32, -1 = off) */
+    int32_t slice_len;         /* GM_MODE_PATTERN: once no root is left to claim, a wave whose current
+                                  root still has more than 4*slice_len depth-2 candidates keeps
+                                  slice_len of them and donates the rest, in slices of slice_len, to
+                                  idle waves (0 = default 32, -1 = off) */
     int32_t reserved[1];
 } gm_tuning;
 
@@ -120,7 +121,9 @@ typedef struct gm_stats {
 } gm_stats;
 
 /* gm_count flags */
-#define GM_COUNT_ALG_BYTES 1 /* also tally the algorithmic byte model (slower) */
+#define GM_COUNT_ALG_BYTES 1      /* also tally the algorithmic byte model (slower) */
+#define GM_COUNT_EAGER_DONATION 2 /* donate depth-2 slices whenever a set is long, not only once the
+                                     roots run out (for tests and tuning; counts are unchanged) */
 
 /* Upload a CSR graph (row_ptr[n+1] int64, col[nnz] int32, rows sorted) to
  * the current device.  max_isect bounds every materialised intersection

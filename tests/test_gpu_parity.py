@@ -440,8 +440,9 @@ def test_root_stride_counts_exactly_the_multiples(stride):
 @pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "c6"])
 def test_slice_donation_keeps_sums_and_bytes(name):
     """Depth-2 slices donated to idle waves (slice lengths 1, 4 and the
-    default; 1 overflows the 2^18-slot queue, so the kept-locally path runs
-    too) give the undonated sum and byte tally, equal to the oracle's."""
+    default, lazily and eagerly; eager slices of 1 overflow the 2^18-slot
+    queue, so the kept-locally path runs too) give the undonated sum and byte
+    tally, equal to the oracle's."""
     n, adj = PATTERNS[name]
     g = N.Graph.rmat(10, 8, 17)
     v, e = g.csr()
@@ -450,16 +451,24 @@ def test_slice_donation_keeps_sums_and_bytes(name):
     s = N.Schedule.build(n, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
     dg = N.DeviceGraph(v, e)
     dp = N.DevicePlan(s.plan_desc())
-    off, st_off = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, slice_len=-1, lane_tail=-1)
+    # Undonated references: with the lane-parallel tail (the oracle's byte
+    # model charges the tail depth's builds as size-only) and without it.
     oraw, ob = O.count_edge_rooted(O.CSR(v, e), s.plan_desc())
-    assert (off, st_off.alg_bytes) == (oraw, ob)
+    ref = {}
+    for tail in (0, -1):
+        got, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, slice_len=-1, lane_tail=tail)
+        ref[tail] = (got, st.alg_bytes)
+    assert ref[0] == (oraw, ob)
+    assert ref[-1][0] == oraw
+    off = oraw
     for sl in (1, 4, 0):
         for tail in (-1, 0):
             print(f"slice_len {sl} lane_tail {tail}", flush=True)
-            got, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, slice_len=sl, lane_tail=tail)
-            assert got == off and st.alg_bytes == ob, (sl, tail)
+            for eager in (0, N.GM_COUNT_EAGER_DONATION):
+                got, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES | eager, slice_len=sl, lane_tail=tail)
+                assert (got, st.alg_bytes) == ref[tail], (sl, tail, eager)
     # captured batch with donation, replayed
-    b = N.Batch(dg, [0, len(e)], [dp], slice_len=1)
+    b = N.Batch(dg, [0, len(e)], [dp], flags=N.GM_COUNT_EAGER_DONATION, slice_len=1)
     for _ in range(3):
         assert b.run()[0][0][0] == off
     b.free()
