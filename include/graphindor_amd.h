@@ -104,7 +104,7 @@ typedef struct gm_tuning {
     int32_t slice_len;         /* GM_MODE_PATTERN: once no root is left to claim, a wave whose current
                                   root still has more than 4*slice_len depth-2 candidates keeps
                                   slice_len of them and donates the rest, in slices of slice_len, to
-                                  idle waves (0 = default 32, -1 = off) */
+                                  idle waves (0 or -1 = off, the default) */
     int32_t reserved[1];
 } gm_tuning;
 
