@@ -370,7 +370,23 @@ std::vector<ScheduleCandidate> Schedule_IEP::enumerate_candidates(const Pattern&
     if (tri_cnt < 0) throw std::logic_error("enumerate_candidates: the triangle count is required");
     const int n = pattern.get_size();
     const int* src = pattern.get_adj_mat_ptr();
-    Schedule_IEP s(src, n);  // scratch whose tables each order rebuilds
+    // Scratch whose tables each order rebuilds.  Its constructor wants an
+    // order in which every vertex touches an earlier one: a BFS order.
+    std::vector<int> bfs(1, 0), seen(n, 0);
+    seen[0] = 1;
+    for (size_t h = 0; h < bfs.size(); ++h)
+        for (int v = 0; v < n; ++v)
+            if (!seen[v] && src[bfs[h] * n + v]) {
+                seen[v] = 1;
+                bfs.push_back(v);
+            }
+    if ((int)bfs.size() != n) throw std::runtime_error("pattern is not connected");
+    std::vector<int> start(n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) start[i * n + j] = src[bfs[i] * n + bfs[j]];
+    Schedule_IEP s(start.data(), n);
+    // Orders below are relabellings of the pattern as given.
+    std::memcpy(s.adj_mat, src, sizeof(int) * n * n);
     std::vector<std::vector<int> > orders = all_permutations(n);
     s.remove_invalid_permutation(orders);
     int deepest = 0;

@@ -87,3 +87,19 @@ def test_encoder_shapes_and_fields():
     assert np.allclose(x[:, 68:80].sum(axis=1), 1.0)          # degree histogram
     # adjacency bits: house has 6 edges
     assert (x[:, :21].sum(axis=1) == 6).all()
+
+
+def test_candidates_of_random_patterns():
+    """Every random training pattern has candidates, each one valid and
+    buildable, whatever the labelling of the pattern."""
+    for n, adj in CM.random_connected_patterns(12, seed=6):
+        cs = N.schedule_candidates(n, adj, 1 << 16, 1 << 20, 1 << 18)
+        assert cs
+        for c in cs[:8]:
+            reordered = "".join(adj[c.order[i] * n + c.order[j]] for i in range(n) for j in range(n))
+            assert N.restrictions_valid(n, reordered, list(c.pairs))
+            N.Schedule.from_candidate(n, adj, c)
+        g = N.Graph.rmat(9, 4, 1)
+        v, e = g.csr()
+        x = N.encode_candidates(n, adj, v, g.info()["tri_cnt"])
+        assert x.shape == (len(cs), 80)
