@@ -68,12 +68,15 @@ def _time_plan(dg, nnz, dp, stride, reps):
     return best
 
 
-def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=None, log=None, budget_s=30.0):
+def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=None, log=None, budget_s=30.0,
+               skip_ms=1000.0):
     """Times candidates of one pattern on one device graph on a shared root
-    sample.  Returns (indices, ms, stride); the analytic pick is always the
-    first.  The sample starts at ~4096 roots and grows (by 8x) until the
-    analytic pick takes at least target_ms / 4; candidates are timed until
-    `budget_s` is spent."""
+    sample.  Returns (indices, ms, stride), the analytic pick first, or None
+    when the group is skipped.  The sample starts at ~1024 roots; a group
+    whose analytic pick already takes more than skip_ms there is skipped (a
+    few hub roots dominate it, so its timings would measure them alone).
+    Otherwise the sample grows (by 4x) until the analytic pick takes at least
+    target_ms / 4, and candidates are timed until `budget_s` is spent."""
     analytic = min(range(len(cands)), key=lambda i: (cands[i].analytic_cost, i))
     idx = list(range(len(cands)))
     if max_cands and len(idx) > max_cands:
@@ -85,10 +88,14 @@ def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=
         idx = [analytic] + idx
     sched = N.Schedule.from_candidate(n, adj, cands[analytic])
     dp = N.DevicePlan(sched.plan_desc())
-    stride = max(1, nnz // 4096)
+    stride = max(1, nnz // 1024)
     t0 = _time_plan(dg, nnz, dp, stride, 1)
+    if t0 > skip_ms:
+        if log:
+            log(f"    {n} {adj}: skipped, analytic pick {t0:.0f} ms on {nnz // stride} roots")
+        return None
     while t0 < target_ms / 4 and stride > 1:
-        stride = max(1, stride // 8)
+        stride = max(1, stride // 4)
         t0 = _time_plan(dg, nnz, dp, stride, 1)
     ms, start = [], time.time()
     for i in idx:
@@ -104,8 +111,9 @@ def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=
     return idx, ms, stride
 
 
-def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, log=print, budget_s=30.0):
+def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, log=None, budget_s=30.0):
     """Dataset over R-MAT graphs (scale, edge_factor, seed) x patterns."""
+    log = log or (lambda *a, **k: print(*a, flush=True))
     rng = np.random.default_rng(seed)
     xs, ts, goff, meta = [], [], [0], []
     for scale, ef, gseed in graph_specs:
@@ -120,7 +128,10 @@ def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, 
             if len(cands) < 2:
                 continue
             x = N.encode_candidates(n, adj, v, inf["tri_cnt"])
-            idx, ms, stride = time_group(dg, len(e), n, adj, cands, target_ms, reps, max_cands, rng, log, budget_s)
+            got = time_group(dg, len(e), n, adj, cands, target_ms, reps, max_cands, rng, log, budget_s)
+            if got is None or len(got[0]) < 2:
+                continue
+            idx, ms, stride = got
             xs.append(x[idx])
             ts.append(np.log(np.maximum(np.asarray(ms, dtype=np.float64), 1e-4)).astype(np.float32))
             goff.append(goff[-1] + len(idx))

@@ -78,6 +78,7 @@ struct Counters {
     unsigned long long tasks;
     unsigned int overflow;
     unsigned int pad;
+    unsigned long long big_cursor; // block-per-root bitmap kernel's vertex cursor
     unsigned long long qhead;      // slice queue: next slot to pop
     unsigned long long qtail;      // slice queue: slots reserved by pushers
     unsigned long long pending;    // slices pushed and not yet finished
@@ -1113,8 +1114,16 @@ struct BmArgs {
     unsigned long long* out;   // the plan's sum cell
     Counters* ctr;
     int32_t stats;
+    int32_t big_min_d;         // roots with big_min_d < d and d * W <= big_cap_words go to the
+    int32_t big_cap_words;     // block-per-root kernel (its rows in LDS); 0: none do
 };
 constexpr int kBmChunk = 16;
+
+// Roots the block-per-root kernel takes (the per-wave kernel skips them).
+__device__ __forceinline__ bool bm_is_big(const BmArgs& a, int32_t d) {
+    const int32_t W = (d + 63) >> 6;
+    return a.big_cap_words > 0 && d > a.big_min_d && d <= a.max_d && d * W <= a.big_cap_words;
+}
 
 struct BmShared {
     uint32_t nbr[kBmMaxD];
@@ -1295,7 +1304,7 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
         for (int32_t v0 = vbeg; v0 < vend; ++v0) {
             int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
             int32_t d = unii((int32_t)(r0 - l0));
-            if (d < K - 1 || d > a.max_d) continue;
+            if (d < K - 1 || d > a.max_d || bm_is_big(a, d)) continue;
             ++roots;
             if (a.stats) wave_bytes += 16ull + 4ull * (unsigned long long)d;
             if (d <= 64) acc += bm_root<1, K>(a, sh, gl_rows, (uint32_t)v0, l0, d, bytes);
@@ -1309,6 +1318,150 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
         atomicAdd(a.out, total);
         atomicAdd(&a.ctr->tasks, roots);
         if (a.stats) atomicAdd(&a.ctr->bytes, tb + wave_bytes);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block-per-root bitmap engine for the large roots.
+//
+// On skewed graphs the roots of out-degree a few hundred hold most of the
+// k-clique work (R-MAT 2^20, k = 5: 0.7% of the roots, ~90% of the word
+// operations), and one wave per root with its rows in global memory leaves
+// the grid waiting on them.  Here a block of kBigThreads takes one root at a
+// time: N(v0) and the d x W rows live in LDS (dynamic, sized at launch),
+// each wave builds rows with its lanes over N(u) (LDS atomic OR), and the
+// local edges (i, j) are handed out one per thread from an LDS counter, so a
+// few heavy pairs cannot hold a wave.  Same arithmetic and byte charge as
+// bm_root.
+// ---------------------------------------------------------------------------
+constexpr int kBigThreads = 512;
+constexpr int kBigChunk = 256;  // vertices per cursor claim
+
+template <int K>
+__global__ void __launch_bounds__(kBigThreads) clique_bitmap_big_kernel(BmArgs a) {
+    extern __shared__ unsigned long long big_rows[];
+    __shared__ uint32_t nbr[kBmMaxD];
+    __shared__ int32_t off[kBmMaxD + 1];
+    __shared__ int32_t pair_ctr;
+    __shared__ long long chunk_sh;
+    __shared__ unsigned long long red[kBigThreads / kWave][2];
+    const int tid = threadIdx.x, ln = tid & (kWave - 1), wv = tid / kWave;
+    constexpr int nwv = kBigThreads / kWave;
+    unsigned long long acc = 0, bytes = 0, roots = 0;
+    const int64_t nchunks = ((int64_t)a.nv + kBigChunk - 1) / kBigChunk;
+    while (true) {
+        if (tid == 0) chunk_sh = (long long)atomicAdd(&a.ctr->big_cursor, 1ull);
+        __syncthreads();
+        const long long chunk = chunk_sh;
+        __syncthreads();
+        if (chunk >= nchunks) break;
+        const int32_t vbeg = (int32_t)(chunk * kBigChunk);
+        const int32_t vend = vbeg + kBigChunk < a.nv ? vbeg + kBigChunk : a.nv;
+        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
+            const int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
+            const int32_t d = (int32_t)(r0 - l0);
+            if (d < K - 1 || !bm_is_big(a, d)) continue;
+            const int W = (d + 63) >> 6;
+            if (tid == 0) {
+                ++roots;
+                if (a.stats) bytes += 16ull + 4ull * (unsigned long long)d;
+            }
+            for (int i = tid; i < d; i += kBigThreads) nbr[i] = a.col[l0 + i];
+            for (int t = tid; t < d * W; t += kBigThreads) big_rows[t] = 0ull;
+            __syncthreads();
+            // Rows: wave wv builds rows wv, wv + nwv, ...; its lanes split N(u).
+            for (int i = wv; i < d; i += nwv) {
+                const uint32_t u = nbr[i];
+                const int64_t lu = a.row[u], ru = a.row[u + 1];
+                const int32_t du = (int32_t)(ru - lu);
+                if (ln == 0 && a.stats) bytes += 8ull + 4ull * (unsigned long long)du;
+                for (int32_t t = ln; t < du; t += kWave) {
+                    const uint32_t x = a.col[lu + t];
+                    int32_t base = 0, n = d;
+                    while (n > 1) {
+                        const int32_t half = n >> 1;
+                        base = (nbr[base + half] <= x) ? base + half : base;
+                        n -= half;
+                    }
+                    if (nbr[base] == x) atomicOr(&big_rows[i * W + (base >> 6)], 1ull << (base & 63));
+                }
+            }
+            __syncthreads();
+            if constexpr (K == 3) {
+                for (int t = tid; t < d * W; t += kBigThreads) acc += __popcll(big_rows[t]);
+                __syncthreads();
+                continue;
+            } else {
+                // Row popcounts, then an exclusive scan by wave 0.
+                for (int i = tid; i < d; i += kBigThreads) {
+                    int32_t c = 0;
+                    for (int w = 0; w < W; ++w) c += __popcll(big_rows[i * W + w]);
+                    off[i] = c;
+                }
+                __syncthreads();
+                if (wv == 0) {
+                    int32_t carry = 0;
+                    for (int base = 0; base < d; base += kWave) {
+                        const int i = base + ln;
+                        const int32_t c = i < d ? off[i] : 0;
+                        int32_t incl = c;
+#pragma unroll
+                        for (int o = 1; o < kWave; o <<= 1) {
+                            const int32_t y = __shfl_up(incl, o);
+                            if (ln >= o) incl += y;
+                        }
+                        if (i < d) off[i] = carry + incl - c;
+                        carry += __shfl(incl, kWave - 1);
+                    }
+                    if (ln == 0) {
+                        off[d] = carry;
+                        pair_ctr = 0;
+                    }
+                }
+                __syncthreads();
+                const int32_t P = off[d];
+                while (true) {
+                    const int32_t pidx = atomicAdd(&pair_ctr, 1);
+                    if (pidx >= P) break;
+                    int32_t lo = 0, n = d;
+                    while (n > 1) {
+                        const int32_t half = n >> 1;
+                        lo = (off[lo + half] <= pidx) ? lo + half : lo;
+                        n -= half;
+                    }
+                    const int i = lo;
+                    int r = pidx - off[i];
+                    int j = 0;
+                    for (int w = 0; w < W; ++w) {
+                        const unsigned long long word = big_rows[i * W + w];
+                        const int c = __popcll(word);
+                        if (r < c) {
+                            j = w * 64 + select_bit64(word, r);
+                            break;
+                        }
+                        r -= c;
+                    }
+                    acc += bm_count_pair<kBmMaxW, K>(big_rows, W, i, j);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    const unsigned long long wa = wave_sum(acc), wb = a.stats ? wave_sum(bytes) : 0ull;
+    if (ln == 0) {
+        red[wv][0] = wa;
+        red[wv][1] = wb;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long ta = 0, tb = 0;
+        for (int k = 0; k < nwv; ++k) {
+            ta += red[k][0];
+            tb += red[k][1];
+        }
+        atomicAdd(a.out, ta);
+        atomicAdd(&a.ctr->tasks, roots);
+        if (a.stats) atomicAdd(&a.ctr->bytes, tb);
     }
 }
 

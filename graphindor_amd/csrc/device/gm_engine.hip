@@ -164,6 +164,35 @@ int pattern_tail_depth(const DevPlan& h) {
 // residency at 10 blocks per CU).
 int bm_blocks(int cus) { return cus * 10; }
 
+// Block-per-root kernel: rows of d x W words in dynamic LDS, up to what a
+// block may allocate beside the kernel's static LDS.
+constexpr int kBigMinD = 192;
+
+hipError_t big_rows_capacity(int device, int32_t* cap_words, size_t* dyn_bytes) {
+    int maxsh = 0;
+    hipError_t e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (e != hipSuccess) return e;
+    const size_t static_bytes = sizeof(uint32_t) * kBmMaxD + sizeof(int32_t) * (kBmMaxD + 1) + 64 +
+                                sizeof(unsigned long long) * 2 * (kBigThreads / kWave) + 64;
+    const size_t avail = (size_t)maxsh > static_bytes ? (size_t)maxsh - static_bytes : 0;
+    const size_t words = std::min<size_t>(avail / sizeof(unsigned long long), (size_t)kBmMaxD * kBmMaxW);
+    *cap_words = (int32_t)words;
+    *dyn_bytes = words * sizeof(unsigned long long);
+    return hipSuccess;
+}
+
+hipError_t launch_bitmap_big(int k, int blocks, size_t dyn, const BmArgs& b, hipStream_t s) {
+    switch (k) {
+        case 3: hipLaunchKernelGGL((clique_bitmap_big_kernel<3>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
+        case 4: hipLaunchKernelGGL((clique_bitmap_big_kernel<4>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
+        case 5: hipLaunchKernelGGL((clique_bitmap_big_kernel<5>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
+        case 6: hipLaunchKernelGGL((clique_bitmap_big_kernel<6>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
+        case 7: hipLaunchKernelGGL((clique_bitmap_big_kernel<7>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_bitmap(int k, int blocks, const BmArgs& b, hipStream_t s) {
     switch (k) {
         case 3: hipLaunchKernelGGL((clique_bitmap_kernel<kBmMaxW, 3>), dim3(blocks), dim3(kWave), 0, s, b); break;
@@ -603,10 +632,15 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             b.out = cell;
             b.ctr = ctr_bm;
             b.stats = stats_on;
+            size_t big_dyn = 0;
+            b.big_min_d = (t && t->bitmap_block_degree > 0) ? t->bitmap_block_degree : kBigMinD;
+            b.big_cap_words = 0;
+            if (!(t && t->bitmap_block_degree < 0)) HIP_TRY(big_rows_capacity(g->device, &b.big_cap_words, &big_dyn));
             hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_bm,
                                (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
             HIP_TRY(hipEventRecord(dc.evpool[evb], s));
             HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
+            if (b.big_cap_words > 0) HIP_TRY(launch_bitmap_big(p->host.n, cus, big_dyn, b, s));
             HIP_TRY(hipEventRecord(dc.evpool[evb + 1], s));
             ev_bitmap.push_back(evb);
             hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr_bm);

@@ -55,7 +55,7 @@ class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
                 ("lane_tail", ctypes.c_int32), ("root_stride", ctypes.c_int32), ("slice_len", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("bitmap_block_degree", ctypes.c_int32)]
 
 
 class GmStats(ctypes.Structure):
@@ -521,8 +521,10 @@ def count(dgraph, dplan, flags=0, **tuning):
     return raw.value, st
 
 
-def _tuning(waves_per_cu=0, bitmap_max_degree=0, lane_tail=0, chunk_edges=0, root_stride=0, slice_len=0):
+def _tuning(waves_per_cu=0, bitmap_max_degree=0, lane_tail=0, chunk_edges=0, root_stride=0, slice_len=0,
+            bitmap_block_degree=0):
     tun = GmTuning()
+    tun.bitmap_block_degree = bitmap_block_degree
     tun.root_stride = root_stride
     tun.slice_len = slice_len
     tun.waves_per_cu = waves_per_cu

@@ -105,7 +105,9 @@ typedef struct gm_tuning {
                                   root still has more than 4*slice_len depth-2 candidates keeps
                                   slice_len of them and donates the rest, in slices of slice_len, to
                                   idle waves (0 or -1 = off, the default) */
-    int32_t reserved[1];
+    int32_t bitmap_block_degree; /* GM_MODE_CLIQUE: bitmap roots of out-degree above this whose rows fit
+                                  in one block's LDS are counted by a whole block each (0 = default
+                                  192, -1 = off: every bitmap root by one wave) */
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */

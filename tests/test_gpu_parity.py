@@ -259,8 +259,13 @@ def test_bitmap_threshold_does_not_change_counts(dmax):
         s = N.Schedule.plain(k, clique_adj(k))
         s.redundancy = 1
         desc = s.plan_desc(N.GM_MODE_CLIQUE)
-        raw, st = N.count(dg, N.DevicePlan(desc), flags=N.GM_COUNT_ALG_BYTES, bitmap_max_degree=dmax)
-        assert (raw, st.alg_bytes) == O.count_clique_hybrid(csr, desc, dmax)
+        want = O.count_clique_hybrid(csr, desc, dmax)
+        # bitmap roots above the block degree are counted by a whole block
+        # each (default 192, 8: almost all, -1: none)
+        for block in (0, 8, -1):
+            raw, st = N.count(dg, N.DevicePlan(desc), flags=N.GM_COUNT_ALG_BYTES, bitmap_max_degree=dmax,
+                              bitmap_block_degree=block)
+            assert (raw, st.alg_bytes) == want, (k, block)
 
 
 def test_bitmap_wide_rows():
@@ -271,10 +276,14 @@ def test_bitmap_wide_rows():
     dv, de = d.csr()
     dv, de = dv.copy(), de.copy()
     assert np.diff(dv).max() > 256
-    for k in (3, 4):
+    dg = N.DeviceGraph(dv, de)
+    for k in (3, 4, 5):
         s = N.Schedule.plain(k, clique_adj(k))
         s.redundancy = 1
-        gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)
+        want = gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)  # wide roots by a block each, rows in LDS
+        # the same roots one wave each, rows in global scratch
+        raw, _ = N.count(dg, N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE)), bitmap_block_degree=-1)
+        assert raw == want
 
 
 @pytest.mark.parametrize("name", ["house", "house_cfg", "c5", "k4_pendant", "c6", "prism", "diamond", "tailed_triangle"])
