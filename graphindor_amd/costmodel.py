@@ -60,10 +60,15 @@ def random_connected_patterns(count, sizes=(5, 6), seed=6):
     return out
 
 
-def _time_plan(dg, nnz, dp, stride, reps):
+def _time_plan(dg, nnz, dp, stride, reps, limit_ms=0):
+    """Best of `reps` sampled runs in ms; None if one passes limit_ms (the
+    kernel stops itself at the limit, see gm_tuning.time_limit_ms)."""
     best = math.inf
     for _ in range(reps):
-        _, st = N.count_batch(dg, [0, nnz], [dp], root_stride=stride)
+        try:
+            _, st = N.count_batch(dg, [0, nnz], [dp], root_stride=stride, time_limit_ms=int(limit_ms))
+        except N.GmTimeout:
+            return None
         best = min(best, st.kernel_ms)
     return best
 
@@ -89,21 +94,27 @@ def time_group(dg, nnz, n, adj, cands, target_ms=4.0, reps=2, max_cands=96, rng=
     sched = N.Schedule.from_candidate(n, adj, cands[analytic])
     dp = N.DevicePlan(sched.plan_desc())
     stride = max(1, nnz // 1024)
-    t0 = _time_plan(dg, nnz, dp, stride, 1)
-    if t0 > skip_ms:
+    t0 = _time_plan(dg, nnz, dp, stride, 1, skip_ms)
+    if t0 is None:
         if log:
-            log(f"    {n} {adj}: skipped, analytic pick {t0:.0f} ms on {nnz // stride} roots")
+            log(f"    {n} {adj}: skipped, analytic pick over {skip_ms:.0f} ms on {nnz // stride} roots")
         return None
     while t0 < target_ms / 4 and stride > 1:
         stride = max(1, stride // 4)
-        t0 = _time_plan(dg, nnz, dp, stride, 1)
+        t0 = _time_plan(dg, nnz, dp, stride, 1, skip_ms)
+        if t0 is None:
+            return None
+    # A candidate slower than 10x the analytic pick is stopped there and
+    # recorded at that bound.
+    limit = max(50.0, 10.0 * t0)
     ms, start = [], time.time()
     for i in idx:
         if ms and time.time() - start > budget_s:
             break
         sched = N.Schedule.from_candidate(n, adj, cands[i])
         dp = N.DevicePlan(sched.plan_desc())
-        ms.append(_time_plan(dg, nnz, dp, stride, reps))
+        t = _time_plan(dg, nnz, dp, stride, reps, limit)
+        ms.append(limit if t is None else t)
     idx = idx[:len(ms)]
     if log:
         log(f"    {n} {adj}: {len(idx)} candidates, stride {stride}, analytic {ms[0]:.3f} ms, "

@@ -77,7 +77,7 @@ struct Counters {
     unsigned long long bytes;
     unsigned long long tasks;
     unsigned int overflow;
-    unsigned int pad;
+    unsigned int timed_out;        // a wave passed the launch's time limit
     unsigned long long big_cursor; // block-per-root bitmap kernel's vertex cursor
     unsigned long long qhead;      // slice queue: next slot to pop
     unsigned long long qtail;      // slice queue: slots reserved by pushers
@@ -123,6 +123,7 @@ struct KernelArgs {
     int64_t queue_cap;
     int32_t slice_len;        // depth-2 candidates per donated slice
     int32_t eager_donation;   // 1: donate long remainders at once, not only when roots run out
+    int64_t time_limit_ticks; // wall-clock ticks a wave may run (0: no limit)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -195,6 +196,8 @@ struct Wave {
     int32_t cur_pl, cur_gr;        // plan and graph of the current root (for donated slices)
     int64_t cur_e;                 // current root edge
     int64_t nitems;                // (root, plan) work items of the launch
+    uint64_t t_start;              // wall clock at the wave's start
+    int32_t aborted;               // 1: the time limit passed; unwind
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -204,6 +207,16 @@ __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
     r.len = unii(s.len);
     r.pad = 0;
     return r;
+}
+
+// Time limit: true (and the wave marked aborted) once the wave has run
+// longer than the launch allows.  Uniform: every lane reads the same clock.
+__device__ __forceinline__ bool out_of_time(Wave& w) {
+    if (w.aborted) return true;
+    if (w.a.time_limit_ticks <= 0) return false;
+    const uint64_t now = uni64((int64_t)wall_clock64());
+    if ((int64_t)(now - w.t_start) > w.a.time_limit_ticks) w.aborted = 1;
+    return w.aborted != 0;
 }
 
 __device__ __forceinline__ void put_set(Wave& w, int pid, const uint32_t* ptr, int32_t len) {
@@ -684,6 +697,7 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     const TailCtx& c = w.lds->tail;
     unsigned long long acc = 0, bytes = 0;
     for (int32_t b0 = 0; b0 < loop.len; b0 += kWave) {
+        if (out_of_time(w)) break;
         const int32_t i = b0 + ln;
         const bool in_range = i < loop.len;
         const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
@@ -821,6 +835,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
             for (int j = 0; j < cnum; ++j) {
                 uint32_t v = rdlane(cand, j);
                 if (v >= bound) return;  // sorted: nothing further qualifies
+                if (out_of_time(w)) return;
                 if (w.a.stats) w.bytes += 4ull;
                 if (MODE == GM_MODE_PATTERN && in_emb<D>(w, v)) continue;
                 int64_t l = rdlane64(rb, j), r = rdlane64(re, j);
@@ -953,6 +968,8 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     w.lane_sum = 0;
     w.lane_bytes = 0;
     w.overflow = 0;
+    w.aborted = 0;
+    w.t_start = (uint64_t)uni64((int64_t)wall_clock64());
 #pragma unroll
     for (int i = 0; i <= GM_MAX_PATTERN; ++i) w.emb[i] = 0;
     // A set that is never built reads as empty.
@@ -975,6 +992,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     int32_t cur_g = 0, cur_p = 0;
     bool main_done = false;
     while (true) {
+        if (out_of_time(w)) break;
         // Donated slices first: they belong to roots already under way.
         if (a.queue) {
             const int64_t slot = pop_slice(a);
@@ -1075,6 +1093,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
         atomicAdd(&a.ctr->tasks, roots);
         if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes + lane_bytes);
         if (w.overflow) atomicOr(&a.ctr->overflow, 1u);
+        if (w.aborted) atomicOr(&a.ctr->timed_out, 1u);
     }
 }
 

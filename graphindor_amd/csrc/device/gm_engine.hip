@@ -587,6 +587,11 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
         }
     }
+    if (t && t->time_limit_ms > 0) {
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device));
+        a.time_limit_ticks = (int64_t)t->time_limit_ms * (int64_t)std::max(khz, 1);
+    }
     // Sampling is absolute: the counted roots are the edges e of the range
     // with e % root_stride == 0, so splitting a range never changes the sample.
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
@@ -685,6 +690,12 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         bitmap_ms += x;
     }
     if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
+    if (c.timed_out) {
+        // Slices left in the queue keep their ready flags: clear them so the
+        // next launch starts from an all-clear queue.
+        if (a.queue) HIP_TRY(hipMemset(a.queue, 0, sizeof(SliceItem) * kQueueCap));
+        return fail(GM_ERR_TIMEOUT, "gm_count: the time limit passed before the count finished");
+    }
     // Scatter back to raw_sums[graph * np + plan] in the caller's plan order.
     for (int gi = 0; gi < ng; ++gi) {
         for (int k = 0; k < np_pat; ++k) raw_sums[(int64_t)gi * np + order[k]] = hs[(int64_t)gi * np_pat + k];

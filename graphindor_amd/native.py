@@ -19,6 +19,7 @@ GM_ERR_INVALID = -1
 GM_ERR_HIP = -2
 GM_ERR_OOM = -3
 GM_ERR_UNSUPPORTED = -4
+GM_ERR_TIMEOUT = -5
 GM_MODE_PATTERN = 0
 GM_MODE_CLIQUE = 1
 GM_COUNT_ALG_BYTES = 1
@@ -55,7 +56,7 @@ class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
                 ("lane_tail", ctypes.c_int32), ("root_stride", ctypes.c_int32), ("slice_len", ctypes.c_int32),
-                ("bitmap_block_degree", ctypes.c_int32)]
+                ("bitmap_block_degree", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32)]
 
 
 class GmStats(ctypes.Structure):
@@ -195,7 +196,13 @@ def device_count():
     return gm().gm_device_count()
 
 
+class GmTimeout(GmError):
+    """gm_tuning.time_limit_ms passed before the count finished."""
+
+
 def _gm_check(rc, what):
+    if rc == GM_ERR_TIMEOUT:
+        raise GmTimeout(f"{what}: {gm().gm_last_error().decode()}")
     if rc != GM_OK:
         raise GmError(f"{what} failed ({rc}): {gm().gm_last_error().decode()}")
 
@@ -522,8 +529,9 @@ def count(dgraph, dplan, flags=0, **tuning):
 
 
 def _tuning(waves_per_cu=0, bitmap_max_degree=0, lane_tail=0, chunk_edges=0, root_stride=0, slice_len=0,
-            bitmap_block_degree=0):
+            bitmap_block_degree=0, time_limit_ms=0):
     tun = GmTuning()
+    tun.time_limit_ms = time_limit_ms
     tun.bitmap_block_degree = bitmap_block_degree
     tun.root_stride = root_stride
     tun.slice_len = slice_len

@@ -37,6 +37,7 @@ extern "C" {
 #define GM_ERR_HIP (-2)         /* HIP runtime error (message says which) */
 #define GM_ERR_OOM (-3)         /* device allocation failed */
 #define GM_ERR_UNSUPPORTED (-4) /* pattern too large, no device, ... */
+#define GM_ERR_TIMEOUT (-5)     /* gm_tuning.time_limit_ms passed; the sums are incomplete */
 
 /* Largest pattern the kernels enumerate; the reference GPU DFS stops at
  * MAX_DEPTH = 7 (gpu/function/pattern_matching.cuh:7). */
@@ -108,6 +109,9 @@ typedef struct gm_tuning {
     int32_t bitmap_block_degree; /* GM_MODE_CLIQUE: bitmap roots of out-degree above this whose rows fit
                                   in one block's LDS are counted by a whole block each (0 = default
                                   192, -1 = off: every bitmap root by one wave) */
+    int32_t time_limit_ms;     /* GM_MODE_PATTERN / list engine: waves stop once they have run this long
+                                  and the call fails with GM_ERR_TIMEOUT (0 = no limit; for timing
+                                  harnesses that must not wait on a slow candidate) */
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */

@@ -77,3 +77,26 @@ def test_gpu_mc_four_motifs(tmp_path):
         s = N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
         want.append(O.count_edge_rooted(csr, s.plan_desc())[0] // s.redundancy)
     assert counts == want
+
+
+@pytest.mark.gpu
+def test_gpu_graph_with_cost_model(wiki_vote_path, tmp_path):
+    """GRAPHSET_COST_MODEL selects the schedule with a cost model (here an
+    untrained one: any pick is a valid schedule) and the count is unchanged;
+    an unusable file falls back to the analytic pick."""
+    require_gpu()
+    n, adj = PATTERNS["house"]
+    base = run("gpu_graph", wiki_vote_path, str(n), adj)
+    assert base.returncode == 0, base.stderr
+    want = [l for l in base.stdout.splitlines() if l.startswith("Pattern count:")]
+    path = tmp_path / "cm.gscm"
+    N.CostModel.new(64, 11).save(path)
+    r = run("gpu_graph", wiki_vote_path, str(n), adj, env={"GRAPHSET_COST_MODEL": str(path)})
+    assert r.returncode == 0, r.stderr
+    assert "Schedule: learned pick" in r.stderr
+    assert [l for l in r.stdout.splitlines() if l.startswith("Pattern count:")] == want
+    bad = tmp_path / "bad.gscm"
+    bad.write_bytes(b"not a model")
+    r = run("gpu_graph", wiki_vote_path, str(n), adj, env={"GRAPHSET_COST_MODEL": str(bad)})
+    assert r.returncode == 0 and "analytic pick" in r.stderr
+    assert [l for l in r.stdout.splitlines() if l.startswith("Pattern count:")] == want

@@ -481,3 +481,30 @@ def test_slice_donation_keeps_sums_and_bytes(name):
     for _ in range(3):
         assert b.run()[0][0][0] == off
     b.free()
+
+
+def test_time_limit_fails_cleanly_and_leaves_no_state():
+    """A count that passes its time limit fails with GM_ERR_TIMEOUT (also
+    with slices waiting in the donation queue), and the next counts on the
+    same device are exact."""
+    g = N.Graph.rmat(14, 16, 4)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    s = N.Schedule.build(5, PATTERNS["c5"][1], inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
+    dp = N.DevicePlan(s.plan_desc())
+    for extra in ({}, {"slice_len": 4}):
+        flags = N.GM_COUNT_EAGER_DONATION if extra else 0
+        with pytest.raises(N.GmTimeout):
+            N.count(dg, dp, flags=flags, time_limit_ms=5, **extra)
+    small = N.Graph.rmat(10, 8, 17)
+    sv, se = small.csr()
+    sv, se = sv.copy(), se.copy()
+    sinf = small.info()
+    ss = N.Schedule.build(5, PATTERNS["c5"][1], sinf["v_cnt"], sinf["e_cnt"], sinf["tri_cnt"])
+    want = O.count_edge_rooted(O.CSR(sv, se), ss.plan_desc())[0]
+    sdg, sdp = N.DeviceGraph(sv, se), N.DevicePlan(ss.plan_desc())
+    assert N.count(sdg, sdp)[0] == want
+    assert N.count(sdg, sdp, flags=N.GM_COUNT_EAGER_DONATION, slice_len=4)[0] == want
+    assert N.count(sdg, sdp, time_limit_ms=60000)[0] == want
