@@ -29,11 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of clique_bitmap_kernel<5> on the default workload
-# (R-MAT 2^20 ef16 seed 2, degree-oriented), from two rocprofv3 PMC runs
-# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 1 GiB device copy
-# in the same runs); see profiles/clique5_rmat20_hbm_traffic.md.
-TRAFFIC_BITMAP5_RMAT20 = 36.12e9
+# HBM bytes per launch of the dominant kernel on the default workload (R-MAT
+# 2^20 ef16 seed 2, degree-oriented), from two rocprofv3 PMC runs (FETCH_SIZE
+# and WRITE_SIZE, each scaled by what it reads for a 1 GiB device copy in the
+# same run); see profiles/clique5_rmat20_hbm_traffic.md.
+TRAFFIC = {}
 
 
 def log(*a):
@@ -54,10 +54,10 @@ def time_counts(N, dg, plan, steps, warmup):
     for i in range(max(1, warmup)):
         r, st = N.count(dg, plan, flags=N.GM_COUNT_ALG_BYTES if i == 0 else 0)
         if i == 0:
-            tally = (st.alg_bytes, st.bitmap_alg_bytes)
+            tally = (st.alg_bytes, st.bitmap_alg_bytes, st.bitmap_block_alg_bytes)
         raw = r if raw is None else raw
         assert r == raw
-    walls, kms, lms, bms = [], [], [], []
+    walls, kms, lms, bms, kbs = [], [], [], [], []
     for _ in range(steps):
         t0 = time.perf_counter()
         r, st = N.count(dg, plan)
@@ -65,10 +65,11 @@ def time_counts(N, dg, plan, steps, warmup):
         kms.append(st.kernel_ms)
         lms.append(st.list_ms)
         bms.append(st.bitmap_ms)
+        kbs.append(st.bitmap_block_ms)
         assert r == raw, "count changed between steps"
     mean = lambda x: sum(x) / len(x)
-    return dict(count=raw, alg_bytes=tally[0], bitmap_alg_bytes=tally[1], wall_s=mean(walls), kernel_ms=mean(kms),
-                list_ms=mean(lms), bitmap_ms=mean(bms))
+    return dict(count=raw, alg_bytes=tally[0], bitmap_alg_bytes=tally[1], block_alg_bytes=tally[2],
+                wall_s=mean(walls), kernel_ms=mean(kms), list_ms=mean(lms), bitmap_ms=mean(bms), block_ms=mean(kbs))
 
 
 def cpu_baseline(N, O, v, e, k, target_s=20.0):
@@ -177,23 +178,35 @@ def pm_leg(N, scale=22, seed=4, budget_s=8.0):
         row = {}
         sa, ia = N.select_schedule(5, adj, v, inf["tri_cnt"], None)
         pa = N.DevicePlan(sa.plan_desc())
+        limit = int(budget_s * 1000)
         S = 1 << 16
-        while True:
-            _, st = N.count_batch(dg, [0, len(e)], [pa], root_stride=S)
-            if st.kernel_ms / 1e3 >= budget_s / 8 or S == 1:
-                break
+
+        def timed(plan):
+            try:
+                return N.count_batch(dg, [0, len(e)], [plan], root_stride=S, time_limit_ms=limit)[1].kernel_ms
+            except N.GmTimeout:
+                return None
+        t = timed(pa)
+        while t is not None and t / 1e3 < budget_s / 8 and S > 1:
             S //= 2
-        _, st = N.count_batch(dg, [0, len(e)], [pa], root_stride=S)
-        row.update({"root_stride": S, "analytic_ms": round(st.kernel_ms, 3),
-                    "analytic_candidate": ia["analytic"], "candidates": ia["candidates"],
+            t = timed(pa)
+        row.update({"root_stride": S, "analytic_candidate": ia["analytic"], "candidates": ia["candidates"],
                     "select_ms_analytic": round(ia["select_ms"], 2)})
-        if model is not None:
+        if t is None:
+            # a few hub roots of the smallest sample already exceed the budget
+            row["analytic_ms"] = f"over {limit} ms"
+            out["patterns"][name] = row
+            log(f"pm {name}: stride {S}: over the {limit} ms budget")
+            continue
+        t = timed(pa)
+        row["analytic_ms"] = round(t, 3) if t is not None else f"over {limit} ms"
+        if model is not None and t is not None:
             sl, il = N.select_schedule(5, adj, v, inf["tri_cnt"], model)
-            pl = N.DevicePlan(sl.plan_desc())
-            _, st2 = N.count_batch(dg, [0, len(e)], [pl], root_stride=S)
-            row.update({"learned_ms": round(st2.kernel_ms, 3), "learned_candidate": il["chosen"],
-                        "select_ms_learned": round(il["select_ms"], 2),
-                        "learned_speedup": round(st.kernel_ms / max(st2.kernel_ms, 1e-6), 3)})
+            t2 = timed(N.DevicePlan(sl.plan_desc()))
+            row.update({"learned_ms": round(t2, 3) if t2 is not None else f"over {limit} ms",
+                        "learned_candidate": il["chosen"], "select_ms_learned": round(il["select_ms"], 2)})
+            if t2 is not None:
+                row["learned_speedup"] = round(t / max(t2, 1e-6), 3)
         out["patterns"][name] = row
         log(f"pm {name}: stride {S}: {row}")
     out["gen_s"] = round(gen_s, 1)
@@ -266,8 +279,8 @@ def main():
         r = time_counts(N, dg, plan, steps, args.warmup if k == 5 else 1)
         results[k] = r
         log(f"{k}-clique: {r['count']} in {r['wall_s']:.4f} s (region {r['kernel_ms']:.2f} ms: bitmap engine "
-            f"{r['bitmap_ms']:.2f} ms, list engine {r['list_ms']:.2f} ms), alg bytes {r['alg_bytes']} "
-            f"(bitmap {r['bitmap_alg_bytes']})")
+            f"{r['bitmap_ms']:.2f} ms of which block-per-root {r['block_ms']:.2f} ms, list engine {r['list_ms']:.2f} ms), "
+            f"alg bytes {r['alg_bytes']} (bitmap {r['bitmap_alg_bytes']}, block-per-root {r['block_alg_bytes']})")
 
     # The reference's own orientation (edges kept toward smaller ids,
     # src/graph.cpp:710-735), 4-cliques only: its 5-clique count is far
@@ -301,11 +314,13 @@ def main():
     r5 = results[5]
     # Dominant kernel of the 5-clique count: the one with the larger share of
     # the region; its algorithmic bytes over its own HIP-event time.
-    if r5["bitmap_ms"] >= r5["list_ms"]:
-        dom = {"kernel": "clique_bitmap_kernel<5>", "ms": r5["bitmap_ms"], "bytes": r5["bitmap_alg_bytes"]}
-    else:
-        dom = {"kernel": "enumerate_kernel<GM_MODE_CLIQUE>", "ms": r5["list_ms"],
-               "bytes": r5["alg_bytes"] - r5["bitmap_alg_bytes"]}
+    parts = [
+        {"kernel": "clique_bitmap_kernel<16,5>", "ms": r5["bitmap_ms"] - r5["block_ms"],
+         "bytes": r5["bitmap_alg_bytes"] - r5["block_alg_bytes"]},
+        {"kernel": "clique_bitmap_big_kernel<5>", "ms": r5["block_ms"], "bytes": r5["block_alg_bytes"]},
+        {"kernel": "enumerate_kernel<1,5>", "ms": r5["list_ms"], "bytes": r5["alg_bytes"] - r5["bitmap_alg_bytes"]},
+    ]
+    dom = max(parts, key=lambda x: x["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     out = {
         "metric": "5-vertex PM & 5-clique time on 1 MI355X (exact); learned vs plain schedule",
@@ -331,11 +346,10 @@ def main():
                    "er_control": control},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC_BITMAP5_RMAT20 if (args.scale == 20 and dom["kernel"] == "clique_bitmap_kernel<5>")
-                     else None,
+                     "traffic": TRAFFIC.get((args.scale, dom["kernel"])),
                      "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 3), "alg_bytes": dom["bytes"],
                      "region_ms": round(r5["kernel_ms"], 3),
-                     "other_kernel_ms": round(r5["kernel_ms"] - dom["ms"], 3)},
+                     "kernels_ms": {x["kernel"]: round(x["ms"], 3) for x in parts}},
     }
     if not args.no_batch:
         out["config"]["batch_4motif"] = batch_leg(N, 512, args.batch_stride)

@@ -153,24 +153,75 @@ def collect(graph_specs, patterns, target_ms=4.0, reps=2, max_cands=96, seed=6, 
             "meta": meta}
 
 
-def train(ds, hidden=256, steps=3000, lr=1e-3, weight_decay=1e-4, seed=6):
-    m = N.CostModel.new(hidden, seed)
-    loss = m.train(ds["x"], ds["log_ms"], ds["goff"], steps, lr, weight_decay, fit_norm=True)
-    return m, loss
+def subset(ds, groups):
+    """The dataset restricted to the given group indices."""
+    xs, ts, goff = [], [], [0]
+    for g in groups:
+        a, b = ds["goff"][g], ds["goff"][g + 1]
+        xs.append(ds["x"][a:b])
+        ts.append(ds["log_ms"][a:b])
+        goff.append(goff[-1] + (b - a))
+    return {"x": np.concatenate(xs), "log_ms": np.concatenate(ts), "goff": np.asarray(goff, dtype=np.int32)}
 
 
-def group_regret(model, ds):
+def pick(y, margin):
+    """The model's pick in one group (row 0 = the analytic pick): its argmin
+    if that is predicted faster than row 0 by more than `margin` (log time),
+    else row 0 -- the rule CostModel::choose applies."""
+    i = int(np.argmin(y))
+    return i if y[i] < y[0] - margin else 0
+
+
+def group_regret(model, ds, margin=0.0):
     """Per group: time of the model's pick / best time, and of the analytic pick (row 0)."""
     y = model.predict(ds["x"])
     out = []
     for g in range(len(ds["goff"]) - 1):
         a, b = ds["goff"][g], ds["goff"][g + 1]
         t = np.exp(ds["log_ms"][a:b])
-        out.append((float(t[int(np.argmin(y[a:b]))] / t.min()), float(t[0] / t.min())))
+        out.append((float(t[pick(y[a:b], margin)] / t.min()), float(t[0] / t.min())))
     return out
 
 
-def evaluate(model, scale, ef, seed, patterns, log=print, root_stride=1):
+def train(ds, hidden=256, steps=3000, lr=1e-3, weight_decay=1e-4, seed=6, val_frac=0.25, chunk=250, tmp=None,
+          log=print):
+    """Trains on 1 - val_frac of the groups; keeps the checkpoint and the
+    trust margin with the lowest mean regret (time of the pick / best time)
+    on the held-out groups.  Returns (model, info)."""
+    import os
+    import tempfile
+    ng = len(ds["goff"]) - 1
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(ng)
+    nval = max(1, int(round(ng * val_frac))) if ng > 3 else 0
+    val, tr = sorted(perm[:nval].tolist()), sorted(perm[nval:].tolist())
+    dtr = subset(ds, tr)
+    dval = subset(ds, val) if val else None
+    margins = [0.0, math.log(1.05), math.log(1.1), math.log(1.25), math.log(1.5), 1e9]
+    m = N.CostModel.new(hidden, seed)
+    path = tmp or os.path.join(tempfile.mkdtemp(), "best.gscm")
+    best = (math.inf, None, None, 0)
+    loss = None
+    for it in range(max(1, steps // chunk)):
+        loss = m.train(dtr["x"], dtr["log_ms"], dtr["goff"], chunk, lr, weight_decay, fit_norm=(it == 0))
+        if dval is None:
+            continue
+        for mg in margins:
+            reg = float(np.mean([r[0] for r in group_regret(m, dval, mg)]))
+            if reg < best[0] - 1e-9:
+                m.margin = mg
+                m.save(path)
+                best = (reg, mg, (it + 1) * chunk, loss)
+    info = {"groups_train": len(tr), "groups_val": len(val), "loss": loss}
+    if dval is not None and best[1] is not None:
+        m = N.CostModel.load(path)
+        analytic = float(np.mean([r[1] for r in group_regret(m, dval, 1e9)]))
+        info.update({"val_regret_learned": best[0], "val_regret_analytic": analytic, "margin": best[1],
+                     "steps": best[2]})
+    return m, info
+
+
+def evaluate(model, scale, ef, seed, patterns, log=print, root_stride=1, limit_ms=60000):
     """Analytic vs learned pick on one R-MAT graph.  root_stride 1: full
     counts, which must agree; S > 1: both schedules timed on the root sample
     e % S == 0 (sampled sums are not counts and differ between schedules)."""
@@ -185,8 +236,17 @@ def evaluate(model, scale, ef, seed, patterns, log=print, root_stride=1):
         for kind, mdl in (("analytic", None), ("learned", model)):
             s, info = N.select_schedule(n, adj, v, inf["tri_cnt"], mdl)
             dp = N.DevicePlan(s.plan_desc())
-            raw, st = N.count_batch(dg, [0, len(e)], [dp], root_stride=root_stride)
-            res[kind] = {"raw": raw[0][0], "redundancy": s.redundancy, "ms": st.kernel_ms, "choice": info}
+            try:
+                raw, st = N.count_batch(dg, [0, len(e)], [dp], root_stride=root_stride, time_limit_ms=limit_ms)
+                res[kind] = {"raw": raw[0][0], "redundancy": s.redundancy, "ms": st.kernel_ms, "choice": info}
+            except N.GmTimeout:
+                res[kind] = {"raw": None, "redundancy": s.redundancy, "ms": float(limit_ms), "choice": info,
+                             "timed_out": True}
+        if any(r.get("timed_out") for r in res.values()):
+            rows.append({"pattern": name, **res})
+            log(f"  {name}: over the {limit_ms} ms limit (analytic {res['analytic']['ms']:.1f} ms, "
+                f"learned {res['learned']['ms']:.1f} ms)")
+            continue
         if root_stride == 1:
             ca = res["analytic"]["raw"] // res["analytic"]["redundancy"]
             cl = res["learned"]["raw"] // res["learned"]["redundancy"]
@@ -210,6 +270,8 @@ def main(argv=None):
     t.add_argument("--max-cands", type=int, default=96)
     t.add_argument("--dataset", default=None, help="also write the timing dataset (.npz)")
     t.add_argument("--budget-s", type=float, default=30.0, help="timing budget per (pattern, graph) group")
+    t.add_argument("--ef", default="16", help="edge factors, one per scale or one for all")
+    t.add_argument("--from-dataset", default=None, help="train on a saved timing dataset (.npz) instead")
     e = sub.add_parser("eval")
     e.add_argument("--model", required=True)
     e.add_argument("--scale", type=int, default=20)
@@ -219,16 +281,24 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "train":
         t0 = time.time()
-        pats = random_connected_patterns(a.patterns, seed=6)
-        specs = [(int(s), 16, 6 + i) for i, s in enumerate(a.scales.split(","))]
-        ds = collect(specs, pats, a.target_ms, 2, a.max_cands, budget_s=a.budget_s)
+        if a.from_dataset:
+            z = np.load(a.from_dataset)
+            ds = {"x": z["x"], "log_ms": z["log_ms"], "goff": z["goff"], "meta": json.loads(str(z["meta"]))}
+        else:
+            pats = random_connected_patterns(a.patterns, seed=6)
+            scales = [int(s) for s in a.scales.split(",")]
+            efs = [int(x) for x in a.ef.split(",")]
+            efs = efs * len(scales) if len(efs) == 1 else efs
+            specs = [(sc, ef, 6 + i) for i, (sc, ef) in enumerate(zip(scales, efs))]
+            ds = collect(specs, pats, a.target_ms, 2, a.max_cands, budget_s=a.budget_s)
         print(f"dataset: {len(ds['log_ms'])} timings in {len(ds['goff']) - 1} groups, {time.time() - t0:.1f} s",
               flush=True)
-        m, loss = train(ds, steps=a.steps)
+        m, info = train(ds, steps=a.steps)
         m.save(a.out)
-        reg = group_regret(m, ds)
-        print(json.dumps({"loss": loss, "train_regret_learned": float(np.mean([r[0] for r in reg])),
-                          "train_regret_analytic": float(np.mean([r[1] for r in reg]))}), flush=True)
+        reg = group_regret(m, ds, m.margin)
+        info.update({"all_regret_learned": float(np.mean([r[0] for r in reg])),
+                     "all_regret_analytic": float(np.mean([r[1] for r in reg]))})
+        print(json.dumps(info), flush=True)
         if a.dataset:
             np.savez(a.dataset, x=ds["x"], log_ms=ds["log_ms"], goff=ds["goff"], meta=json.dumps(ds["meta"]))
     else:

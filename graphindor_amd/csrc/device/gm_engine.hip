@@ -100,6 +100,10 @@ int grid_blocks(int cus, int blocks_per_cu, int waves_per_cu) {
     return std::max(1, cus * w / kWavesPerBlock);
 }
 
+// Counters per call: [0] totals, [1] one clique list launch, [2] one bitmap
+// count, [3] bitmap totals, [4] one block-per-root launch, [5] its totals.
+constexpr int kNumCounters = 6;
+
 struct DeviceCache {
     std::mutex mu;
     void* scratch = nullptr;
@@ -529,19 +533,19 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     HIP_TRY(grow(&dc.plan_ptrs, &dc.plan_ptrs_bytes, sizeof(DevPlan*) * (size_t)np));
     HIP_TRY(grow(&dc.gedge, &dc.gedge_bytes, sizeof(int64_t) * (size_t)(ng + 1)));
     HIP_TRY(grow(&dc.sums, &dc.sums_bytes, sizeof(unsigned long long) * (size_t)ng * (size_t)np));
-    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * 4));
+    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * kNumCounters));
     if (!dc.ev0) {
         HIP_TRY(hipEventCreate(&dc.ev0));
         HIP_TRY(hipEventCreate(&dc.ev1));
     }
     // Event pairs: the fused pattern launch, then a list and a bitmap launch per clique plan.
-    const size_t nev = 2 * (size_t)(1 + 2 * np_clq);
+    const size_t nev = 2 * (size_t)(1 + 3 * np_clq);
     while (dc.evpool.size() < nev) {
         hipEvent_t ev = nullptr;
         HIP_TRY(hipEventCreate(&ev));
         dc.evpool.push_back(ev);
     }
-    std::vector<int> ev_list, ev_bitmap;  // indices of start events used
+    std::vector<int> ev_list, ev_bitmap, ev_block;  // indices of start events used
     std::vector<const DevPlan*> hptrs(np);
     for (int k = 0; k < np; ++k) hptrs[k] = plans[order[k]]->d_plan;
     HIP_TRY(hipMemcpyAsync(dc.plan_ptrs, hptrs.data(), sizeof(DevPlan*) * np, hipMemcpyHostToDevice, s));
@@ -552,7 +556,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     unsigned long long* d_sums = (unsigned long long*)dc.sums;
     const DevPlan* const* d_plans = (const DevPlan* const*)dc.plan_ptrs;
     const int64_t ncells = (int64_t)ng * np;
-    const int64_t nctr_words = (int64_t)(sizeof(Counters) * 4 / sizeof(unsigned long long));
+    const int64_t nctr_words = (int64_t)(sizeof(Counters) * kNumCounters / sizeof(unsigned long long));
 
     HIP_TRY(hipEventRecord(dc.ev0, s));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, s, d_sums, ncells);
@@ -614,8 +618,10 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     Counters* ctr2 = dc.ctr + 1;
     Counters* ctr_bm = dc.ctr + 2;
     Counters* ctr_bm_total = dc.ctr + 3;
+    Counters* ctr_blk = dc.ctr + 4;
+    Counters* ctr_blk_total = dc.ctr + 5;
     for (int c = 0; c < np_clq; ++c) {
-        const int evl = 2 + 4 * c, evb = evl + 2;
+        const int evl = 2 + 6 * c, evb = evl + 2, evk = evl + 4;
         const gm_plan* p = plans[order[np_pat + c]];
         unsigned long long* cell = d_sums + (int64_t)ng * np_pat + c;  // stride 1 per graph below
         const bool use_bitmap = ng == 1 && !graph_edge_offsets && p->host.clique_chain && bm_max > 0;
@@ -645,7 +651,19 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
                                (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
             HIP_TRY(hipEventRecord(dc.evpool[evb], s));
             HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
-            if (b.big_cap_words > 0) HIP_TRY(launch_bitmap_big(p->host.n, cus, big_dyn, b, s));
+            if (b.big_cap_words > 0) {
+                // The large roots: their own counters and events, so their share is reported.
+                hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_blk,
+                                   (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
+                BmArgs bb = b;
+                bb.ctr = ctr_blk;
+                HIP_TRY(hipEventRecord(dc.evpool[evk], s));
+                HIP_TRY(launch_bitmap_big(p->host.n, cus, big_dyn, bb, s));
+                HIP_TRY(hipEventRecord(dc.evpool[evk + 1], s));
+                ev_block.push_back(evk);
+                hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm, ctr_blk);
+                hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_blk_total, ctr_blk);
+            }
             HIP_TRY(hipEventRecord(dc.evpool[evb + 1], s));
             ev_bitmap.push_back(evb);
             hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr_bm);
@@ -671,10 +689,11 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     }
     HIP_TRY(hipEventRecord(dc.ev1, s));
     std::vector<unsigned long long> hs(ncells);
-    Counters c, cb;
+    Counters c, cb, ck;
     HIP_TRY(hipMemcpyAsync(hs.data(), d_sums, sizeof(unsigned long long) * ncells, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&c, dc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&cb, ctr_bm_total, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&ck, ctr_blk_total, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
@@ -688,6 +707,12 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         float x = 0;
         HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
         bitmap_ms += x;
+    }
+    double block_ms = 0;
+    for (int i : ev_block) {
+        float x = 0;
+        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+        block_ms += x;
     }
     if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
     if (c.timed_out) {
@@ -712,6 +737,8 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         stats->list_ms = list_ms;
         stats->bitmap_ms = bitmap_ms;
         stats->bitmap_alg_bytes = cb.bytes;
+        stats->bitmap_block_ms = block_ms;
+        stats->bitmap_block_alg_bytes = ck.bytes;
     }
     return GM_OK;
 }
@@ -884,6 +911,8 @@ int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats)
         stats->list_ms = ms;
         stats->bitmap_ms = 0;
         stats->bitmap_alg_bytes = 0;
+        stats->bitmap_block_ms = 0;
+        stats->bitmap_block_alg_bytes = 0;
     }
     return GM_OK;
 }

@@ -400,6 +400,9 @@ GS_API int gs_costmodel_train(void* h, const float* x, const float* t, const int
     });
 }
 
+GS_API void gs_costmodel_set_margin(void* h, float m) { ((CostModel*)h)->set_margin(m); }
+GS_API float gs_costmodel_margin(void* h) { return ((CostModel*)h)->margin(); }
+
 GS_API int gs_costmodel_predict(void* h, const float* x, int32_t rows, float* y) {
     return guarded([&] {
         std::vector<float> xv(x, x + (int64_t)rows * kFeatures);

@@ -132,6 +132,8 @@ std::unique_ptr<CostModel> CostModel::load(const std::string& path) {
     std::vector<float> p((size_t)gm_mlp_param_count(m->mlp_));
     in.read((char*)p.data(), sizeof(float) * p.size());
     if (!in) return nullptr;
+    float margin = 0.f;
+    if (in.read((char*)&margin, sizeof(float))) m->margin_ = margin;
     if (gm_mlp_set_params(m->mlp_, p.data()) != GM_OK) return nullptr;
     return m;
 }
@@ -148,6 +150,7 @@ void CostModel::save(const std::string& path) const {
     out.write((const char*)mean_.data(), sizeof(float) * F);
     out.write((const char*)std_.data(), sizeof(float) * F);
     out.write((const char*)p.data(), sizeof(float) * p.size());
+    out.write((const char*)&margin_, sizeof(float));
     if (!out) throw std::runtime_error("write failed: " + path);
 }
 
@@ -195,12 +198,15 @@ std::vector<float> CostModel::predict(const std::vector<float>& x, int64_t rows)
     return y;
 }
 
-int CostModel::choose(const Pattern& pattern, const std::vector<ScheduleCandidate>& cands, const GraphStats& gs) {
+int CostModel::choose(const Pattern& pattern, const std::vector<ScheduleCandidate>& cands, const GraphStats& gs,
+                      int fallback) {
     std::vector<float> x = encode_candidates(pattern, cands, gs);
     std::vector<float> y = predict(x, (int64_t)cands.size());
     int best = 0;
     for (size_t i = 1; i < y.size(); ++i)
         if (y[i] < y[best]) best = (int)i;
+    // Trust margin: leave the fallback unless the model predicts a clear gain.
+    if (fallback >= 0 && fallback < (int)y.size() && !(y[best] < y[fallback] - margin_)) return fallback;
     return best;
 }
 
@@ -215,7 +221,7 @@ Schedule_IEP* select_schedule(const Pattern& pattern, const GraphStats& gs, Cost
     int chosen = analytic;
     bool learned = false;
     if (model) {
-        chosen = model->choose(pattern, cands, gs);
+        chosen = model->choose(pattern, cands, gs, analytic);
         learned = true;
     }
     Schedule_IEP* s = new Schedule_IEP(pattern, cands[chosen]);

@@ -23,7 +23,8 @@
 //             (the analytic model's edge and triangle densities), and the
 //             fraction of vertices per log2-degree bucket 0..11
 // Model file: "GSCM0001", int32 F, int32 H, float mean[F], float std[F],
-// float params[gm_mlp_param_count].
+// float params[gm_mlp_param_count], float margin (the trust margin, in log
+// time, that a prediction must beat the analytic pick by).
 #pragma once
 #include <cstdint>
 #include <memory>
@@ -72,8 +73,13 @@ public:
                 int steps, float lr, float weight_decay);
     std::vector<float> predict(const std::vector<float>& x, int64_t rows);
 
-    // Index of the candidate with the smallest predicted run time.
-    int choose(const Pattern& pattern, const std::vector<ScheduleCandidate>& cands, const GraphStats& gs);
+    // Index of the candidate with the smallest predicted run time, unless it
+    // is not predicted faster than `fallback` (the analytic pick) by more
+    // than the margin (in log time), in which case `fallback`.
+    int choose(const Pattern& pattern, const std::vector<ScheduleCandidate>& cands, const GraphStats& gs,
+               int fallback = -1);
+    void set_margin(float m) { margin_ = m; }
+    float margin() const { return margin_; }
 
     int hidden() const { return hidden_; }
 
@@ -83,6 +89,7 @@ private:
     gm_mlp* mlp_ = nullptr;
     int hidden_ = 0;
     std::vector<float> mean_, std_;
+    float margin_ = 0.f;
 };
 
 // How a schedule was selected.

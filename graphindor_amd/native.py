@@ -62,7 +62,8 @@ class GmTuning(ctypes.Structure):
 class GmStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("alg_bytes", ctypes.c_uint64), ("root_tasks", ctypes.c_uint64),
                 ("grid_waves", ctypes.c_int32), ("flags_used", ctypes.c_int32), ("list_ms", ctypes.c_double),
-                ("bitmap_ms", ctypes.c_double), ("bitmap_alg_bytes", ctypes.c_uint64)]
+                ("bitmap_ms", ctypes.c_double), ("bitmap_alg_bytes", ctypes.c_uint64),
+                ("bitmap_block_ms", ctypes.c_double), ("bitmap_block_alg_bytes", ctypes.c_uint64)]
 
 
 _gm = None
@@ -182,6 +183,8 @@ def gs():
             ("gs_costmodel_train", [VP, VP, VP, VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
             ("gs_costmodel_predict", [VP, VP, ctypes.c_int32, VP], ctypes.c_int),
+            ("gs_costmodel_set_margin", [VP, ctypes.c_float], None),
+            ("gs_costmodel_margin", [VP], ctypes.c_float),
             ("gs_schedule_select", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64, VP,
                                     VP, ctypes.POINTER(ctypes.c_double)], VP),
         ]:
@@ -420,6 +423,14 @@ class CostModel:
                                    weight_decay, 1 if fit_norm else 0, ctypes.byref(loss)) != 0:
             raise GmError(gs().gs_last_error().decode())
         return loss.value
+
+    @property
+    def margin(self):
+        return gs().gs_costmodel_margin(self.h)
+
+    @margin.setter
+    def margin(self, m):
+        gs().gs_costmodel_set_margin(self.h, float(m))
 
     def predict(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

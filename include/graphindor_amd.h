@@ -124,6 +124,8 @@ typedef struct gm_stats {
     double list_ms;           /* HIP-event time inside the list-engine launches (enumerate kernel) */
     double bitmap_ms;         /* HIP-event time inside the local-bitmap clique engine launches */
     uint64_t bitmap_alg_bytes;/* the bitmap engine's share of alg_bytes */
+    double bitmap_block_ms;   /* of bitmap_ms: the block-per-root launches (large roots) */
+    uint64_t bitmap_block_alg_bytes; /* of bitmap_alg_bytes: the block-per-root launches */
 } gm_stats;
 
 /* gm_count flags */
