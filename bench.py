@@ -201,7 +201,7 @@ def pm_leg(N, scale=22, seed=4, budget_s=8.0):
         t = timed(pa)
         row["analytic_ms"] = round(t, 3) if t is not None else f"over {limit} ms"
         if model is not None and t is not None:
-            sl, il = N.select_schedule(5, adj, v, inf["tri_cnt"], model)
+            sl, il = N.select_schedule_measured(5, adj, g, model)
             t2 = timed(N.DevicePlan(sl.plan_desc()))
             row.update({"learned_ms": round(t2, 3) if t2 is not None else f"over {limit} ms",
                         "learned_candidate": il["chosen"], "select_ms_learned": round(il["select_ms"], 2)})

@@ -234,7 +234,10 @@ def evaluate(model, scale, ef, seed, patterns, log=print, root_stride=1, limit_m
     for name, n, adj in patterns:
         res = {}
         for kind, mdl in (("analytic", None), ("learned", model)):
-            s, info = N.select_schedule(n, adj, v, inf["tri_cnt"], mdl)
+            if mdl is None:
+                s, info = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
+            else:
+                s, info = N.select_schedule_measured(n, adj, g, mdl)
             dp = N.DevicePlan(s.plan_desc())
             try:
                 raw, st = N.count_batch(dg, [0, len(e)], [dp], root_stride=root_stride, time_limit_ms=limit_ms)

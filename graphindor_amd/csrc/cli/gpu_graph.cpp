@@ -1,8 +1,9 @@
 // gpu_graph <graph.g> <pattern_size> <pattern_adj_string> [1|0 iep]
 // Same arguments and stdout lines as the reference (gpu/gpu_graph.cu:53-114).
-// With GRAPHSET_COST_MODEL=<file> the schedule is the learned cost model's
-// pick among the analytic selector's candidates (the analytic pick if the
-// file is unusable); the choice is reported on stderr.
+// With GRAPHSET_COST_MODEL=<file> the learned cost model proposes schedules
+// among the analytic selector's candidates and a short sampled run of each,
+// and of the analytic pick, decides (the analytic pick if the file is
+// unusable); the choice is reported on stderr.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -62,7 +63,7 @@ int main(int argc, char* argv[]) {
             std::unique_ptr<CostModel> model = CostModel::load(model_path);
             if (!model) fprintf(stderr, "cost model %s unusable: analytic schedule\n", model_path);
             ScheduleChoice ch;
-            owned.reset(select_schedule(p, graph_stats(*g), model.get(), enable_iep != 0, &ch));
+            owned.reset(select_schedule_measured(p, *g, model.get(), enable_iep != 0, MeasureOptions(), &ch));
             fprintf(stderr, "Schedule: %s pick, candidate %d of %d (analytic pick %d), %.3f ms\n",
                     ch.learned ? "learned" : "analytic", ch.chosen, ch.candidates, ch.analytic, ch.select_ms);
         } else {

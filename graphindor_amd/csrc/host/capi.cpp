@@ -412,6 +412,33 @@ GS_API int gs_costmodel_predict(void* h, const float* x, int32_t rows, float* y)
     });
 }
 
+// The model's top_k proposals and the analytic pick, timed on a root sample of
+// graph `gp` (sample_roots roots, time_limit_ms each); the fastest is kept.
+// info = {candidates, chosen, analytic, learned}, *select_ms = time spent.
+GS_API void* gs_schedule_select_measured(int n, const char* adj, int iep, void* gp, void* model, int top_k,
+                                         int sample_roots, int time_limit_ms, int32_t* info, double* select_ms) {
+    void* out = nullptr;
+    guarded([&] {
+        Pattern p(n, adj);
+        MeasureOptions opt;
+        opt.top_k = top_k;
+        opt.sample_roots = sample_roots;
+        opt.time_limit_ms = time_limit_ms;
+        ScheduleChoice ch;
+        Schedule_IEP* s = select_schedule_measured(p, *(Graph*)gp, (CostModel*)model, iep != 0, opt, &ch);
+        out = new GsSchedule{s, FlatPlan(), -1};
+        if (info) {
+            info[0] = ch.candidates;
+            info[1] = ch.chosen;
+            info[2] = ch.analytic;
+            info[3] = ch.learned ? 1 : 0;
+        }
+        if (select_ms) *select_ms = ch.select_ms;
+        return 0;
+    });
+    return out;
+}
+
 // Schedule chosen by `model` (null: the analytic pick) among the candidates;
 // info = {candidates, chosen, analytic pick, learned}, *select_ms = time spent.
 GS_API void* gs_schedule_select(int n, const char* adj, int iep, const int64_t* row_ptr, int32_t nv, int64_t tri_cnt,

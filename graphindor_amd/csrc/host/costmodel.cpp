@@ -9,6 +9,7 @@
 #include <fstream>
 #include <stdexcept>
 
+#include "gpu_plan.h"
 #include "graph.h"
 #include "graphindor_amd.h"
 
@@ -230,6 +231,68 @@ Schedule_IEP* select_schedule(const Pattern& pattern, const GraphStats& gs, Cost
         info->chosen = chosen;
         info->analytic = analytic;
         info->learned = learned;
+        info->select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return s;
+}
+
+Schedule_IEP* select_schedule_measured(const Pattern& pattern, Graph& g, CostModel* model, bool use_iep,
+                                       const MeasureOptions& opt, ScheduleChoice* info) {
+    auto t0 = std::chrono::steady_clock::now();
+    const GraphStats gs = graph_stats(g);
+    std::vector<ScheduleCandidate> cands =
+        Schedule_IEP::enumerate_candidates(pattern, use_iep, (int)gs.v_cnt, gs.nnz, gs.tri_cnt);
+    int analytic = 0;
+    for (size_t i = 1; i < cands.size(); ++i)
+        if (cands[i].analytic_cost < cands[analytic].analytic_cost) analytic = (int)i;
+    std::vector<int> pool(1, analytic);
+    if (model) {
+        std::vector<float> y = model->predict(encode_candidates(pattern, cands, gs), (int64_t)cands.size());
+        std::vector<int> idx(cands.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return y[a] < y[b]; });
+        for (int i : idx) {
+            if ((int)pool.size() > opt.top_k) break;
+            if (i != analytic) pool.push_back(i);
+        }
+    }
+    int chosen = analytic;
+    std::vector<double> ms(pool.size(), 0.0);
+    if (pool.size() > 1) {
+        gm_graph* dg = g.device_graph();
+        const int64_t nnz = g.e_cnt;
+        gm_tuning t{};
+        t.root_stride = (int32_t)std::max<int64_t>(1, nnz / std::max(1, opt.sample_roots));
+        t.time_limit_ms = opt.time_limit_ms;
+        const int64_t range[2] = {0, nnz};
+        double best = 0;
+        for (size_t k = 0; k < pool.size(); ++k) {
+            Schedule_IEP s(pattern, cands[pool[k]]);
+            FlatPlan fp;
+            flatten_schedule(s, GM_MODE_PATTERN, fp);
+            gm_plan* p = nullptr;
+            if (gm_plan_create(&fp.desc, &p) != GM_OK) throw std::runtime_error(gm_last_error());
+            uint64_t raw = 0;
+            gm_stats st{};
+            const int rc = gm_count_batch(dg, range, 1, &p, 1, &t, 0, nullptr, &raw, &st);
+            gm_plan_destroy(p);
+            if (rc == GM_ERR_TIMEOUT) ms[k] = INFINITY;
+            else if (rc != GM_OK) throw std::runtime_error(gm_last_error());
+            else ms[k] = st.kernel_ms;
+            if (k == 0 || ms[k] < best) {
+                best = ms[k];
+                chosen = pool[k];
+            }
+        }
+    }
+    Schedule_IEP* s = new Schedule_IEP(pattern, cands[chosen]);
+    if (info) {
+        info->candidates = (int)cands.size();
+        info->chosen = chosen;
+        info->analytic = analytic;
+        info->learned = model != nullptr;
+        info->measured = pool;
+        info->measured_ms = ms;
         info->select_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return s;

@@ -98,10 +98,27 @@ struct ScheduleChoice {
     int chosen = -1;         // index into the candidate list
     int analytic = -1;       // the analytic model's pick
     bool learned = false;    // false: analytic fallback
-    double select_ms = 0;    // enumeration + encoding + inference
+    double select_ms = 0;    // enumeration + encoding + inference (+ measurement)
+    std::vector<int> measured;        // candidates timed on the root sample
+    std::vector<double> measured_ms;  // their sample times (infinity: over the limit)
+};
+
+// Measurement of the model's proposals: the top_k candidates by predicted
+// time and the analytic pick are counted on the root sample e % S == 0
+// (S = nnz / sample_roots) with a time limit each, and the fastest is kept.
+struct MeasureOptions {
+    int top_k = 3;
+    int sample_roots = 4096;
+    int time_limit_ms = 200;
 };
 
 // Schedule for `pattern` on graph statistics `gs`: the model's pick among the
 // analytic selector's candidates, or the analytic pick when `model` is null.
 Schedule_IEP* select_schedule(const Pattern& pattern, const GraphStats& gs, CostModel* model, bool use_iep,
                               ScheduleChoice* info = nullptr);
+
+// The model proposes, a measurement on `g` (on the current HIP device)
+// decides: the analytic pick is among the measured, so the chosen schedule
+// is never slower than it on the sample.  Without a model: the analytic pick.
+Schedule_IEP* select_schedule_measured(const Pattern& pattern, Graph& g, CostModel* model, bool use_iep,
+                                       const MeasureOptions& opt, ScheduleChoice* info = nullptr);

@@ -184,6 +184,8 @@ def gs():
                                     ctypes.c_int, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
             ("gs_costmodel_predict", [VP, VP, ctypes.c_int32, VP], ctypes.c_int),
             ("gs_costmodel_set_margin", [VP, ctypes.c_float], None),
+            ("gs_schedule_select_measured", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, VP, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, VP, ctypes.POINTER(ctypes.c_double)], VP),
             ("gs_costmodel_margin", [VP], ctypes.c_float),
             ("gs_schedule_select", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64, VP,
                                     VP, ctypes.POINTER(ctypes.c_double)], VP),
@@ -459,6 +461,19 @@ def select_schedule(n, adj, vertex, tri_cnt, model=None, iep=True):
     ms = ctypes.c_double(0)
     h = gs().gs_schedule_select(n, adj.encode(), 1 if iep else 0, v.ctypes.data, len(v) - 1, tri_cnt,
                                 model.h if model is not None else None, info, ctypes.byref(ms))
+    s = Schedule(h)
+    return s, {"candidates": info[0], "chosen": info[1], "analytic": info[2], "learned": bool(info[3]),
+               "select_ms": ms.value}
+
+
+def select_schedule_measured(n, adj, graph, model=None, iep=True, top_k=3, sample_roots=4096, time_limit_ms=200):
+    """(Schedule, info): the model's top_k proposals and the analytic pick
+    timed on a root sample of `graph` (a host Graph; its device copy is
+    used), the fastest kept."""
+    info = (ctypes.c_int32 * 4)()
+    ms = ctypes.c_double(0)
+    h = gs().gs_schedule_select_measured(n, adj.encode(), 1 if iep else 0, graph.h, model.h if model is not None else None,
+                                         top_k, sample_roots, time_limit_ms, info, ctypes.byref(ms))
     s = Schedule(h)
     return s, {"candidates": info[0], "chosen": info[1], "analytic": info[2], "learned": bool(info[3]),
                "select_ms": ms.value}

@@ -153,3 +153,25 @@ def test_cost_model_save_load_and_selection(tmp_path):
         ra, _ = N.count(dg, N.DevicePlan(sa.plan_desc()))
         rl, _ = N.count(dg, N.DevicePlan(sl.plan_desc()))
         assert ra // sa.redundancy == rl // sl.redundancy
+
+
+def test_measured_selection_keeps_counts_and_never_loses_to_analytic_on_its_sample():
+    """The model proposes, a sampled run decides: the chosen schedule counts
+    what the analytic one counts."""
+    g = N.Graph.rmat(10, 4, 5)
+    v, e = g.csr()
+    v, e = v.copy(), e.copy()
+    inf = g.info()
+    dg = N.DeviceGraph(v, e)
+    model = N.CostModel.new(64, 3)
+    for n, adj in [(5, "0101110101010101010011000"), (5, "0100110100010100010110010"),
+                   (6, "011100101010110001100011010101001110")]:
+        sa, ia = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
+        sm, im = N.select_schedule_measured(n, adj, g, model, top_k=3, sample_roots=512, time_limit_ms=2000)
+        assert im["analytic"] == ia["analytic"] and 0 <= im["chosen"] < im["candidates"]
+        ra, _ = N.count(dg, N.DevicePlan(sa.plan_desc()))
+        rm, _ = N.count(dg, N.DevicePlan(sm.plan_desc()))
+        assert ra // sa.redundancy == rm // sm.redundancy
+    # without a model: the analytic pick, no measurement
+    s0, i0 = N.select_schedule_measured(5, "0100110100010100010110010", g, None)
+    assert i0["chosen"] == i0["analytic"] and not i0["learned"]
