@@ -991,6 +991,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     unsigned long long roots = 0;
     int32_t cur_g = 0, cur_p = 0;
     bool main_done = false;
+    int idle_sleeps = 1;
     while (true) {
         if (out_of_time(w)) break;
         // Donated slices first: they belong to roots already under way.
@@ -1019,17 +1020,22 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 const uint32_t u0 = (uint32_t)edge_source(a.row, a.nv, e);
                 process_root<MODE, NMAX>(w, u0, u1, s0, s1, true);
                 if (ln == 0) atomicAdd(&a.ctr->pending, ~0ull);  // one slice done
+                idle_sleeps = 1;
                 continue;
             }
         }
         if (main_done) {
             // Nothing to claim; stop once no donated slice is outstanding.
+            // Idle waves back off exponentially so their polling of the
+            // queue counters does not crowd the L2 channel those live on.
             const unsigned long long pend =
                 a.queue ? __hip_atomic_load(&a.ctr->pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
             if (uni64((int64_t)__shfl((long long)pend, 0)) == 0) break;
-            __builtin_amdgcn_s_sleep(8);
+            for (int k = 0; k < idle_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
+            idle_sleeps = idle_sleeps < 16 ? idle_sleeps * 2 : 16;
             continue;
         }
+        idle_sleeps = 1;
         unsigned long long start = 0;
         int64_t take = 0;
         if (ln == 0) {
@@ -1460,7 +1466,11 @@ __global__ void __launch_bounds__(kBigThreads) clique_bitmap_big_kernel(BmArgs a
                         }
                         r -= c;
                     }
-                    acc += bm_count_pair<kBmMaxW, K>(big_rows, W, i, j);
+                    // Word loops sized to the root's row width (W <= 16).
+                    if (W <= 4) acc += bm_count_pair<4, K>(big_rows, W, i, j);
+                    else if (W <= 8) acc += bm_count_pair<8, K>(big_rows, W, i, j);
+                    else if (W <= 12) acc += bm_count_pair<12, K>(big_rows, W, i, j);
+                    else acc += bm_count_pair<kBmMaxW, K>(big_rows, W, i, j);
                 }
                 __syncthreads();
             }
