@@ -198,6 +198,7 @@ struct Wave {
     int64_t nitems;                // (root, plan) work items of the launch
     uint64_t t_start;              // wall clock at the wave's start
     int32_t aborted;               // 1: the time limit passed; unwind
+    int64_t rebuild_ticks;         // wall-clock ticks the current root's depth-0/1 builds took
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -208,6 +209,10 @@ __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
     r.pad = 0;
     return r;
 }
+
+// Donation only pays for a remainder of at least this many wall-clock
+// ticks (10 ns each at the 100 MHz constant clock: 20 us).
+constexpr int64_t kDonateMinTicks = 2000;
 
 // Time limit: true (and the wave marked aborted) once the wave has run
 // longer than the launch allows.  Uniform: every lane reads the same clock.
@@ -739,11 +744,10 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     wave_sync();
 }
 
-// Pushes slices [lo + slice_len, hi) of the current root's depth-2 loop
-// set, slice_len candidates each, and returns the end of the part this wave
-// keeps.  Slots past the queue's capacity are kept too.
-__device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi) {
-    const int32_t T = w.a.slice_len;
+// Pushes slices [lo + T, hi) of the current root's depth-2 loop set, T
+// candidates each, and returns the end of the part this wave keeps.  Slots
+// past the queue's capacity are kept too.
+__device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi, int32_t T) {
     const int32_t keep_hi = lo + T;
     const int32_t nslices = (hi - keep_hi + T - 1) / T;
     unsigned long long base = 0;
@@ -812,15 +816,34 @@ __device__ __forceinline__ void dfs(Wave& w) {
             lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
             hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
         }
+        uint64_t t_loop0 = 0;
+        if constexpr (D == 2) {
+            if (w.donate) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
+        }
         for (int32_t base = lo; base < hi; base += kWave) {
             if constexpr (D == 2) {
-                // Lazy donation: once no root is left to claim, other waves
-                // would idle, so a long remainder is cut into slices for them.
+                // Donation: once no root is left to claim, a remainder that
+                // would keep this wave busy is cut into slices for idle waves.
+                // Slice length from this root's own costs: a slice's work
+                // (T candidates at the per-candidate time measured so far)
+                // is at least 8x the prefix rebuild a popper pays for it.
                 if (w.donate && hi - base > 4 * w.a.slice_len) {
-                    unsigned long long c = 0;
-                    if (ln == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    c = (unsigned long long)uni64(__shfl((long long)c, 0));
-                    if ((int64_t)c >= w.nitems || w.a.eager_donation) hi = donate_slices(w, base, hi);
+                    int32_t T = w.a.slice_len;
+                    bool go = w.a.eager_donation != 0;
+                    if (!go && base > lo) {
+                        unsigned long long c = 0;
+                        if (ln == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        c = (unsigned long long)uni64(__shfl((long long)c, 0));
+                        if ((int64_t)c >= w.nitems) {
+                            const int64_t el = uni64((int64_t)wall_clock64()) - (int64_t)t_loop0;
+                            const int64_t per = el / (base - lo) + 1;  // ticks per candidate
+                            const int64_t rest = per * (int64_t)(hi - base);
+                            const int64_t want = (8 * w.rebuild_ticks) / per + 1;
+                            T = (int32_t)(want < T ? T : (want > (hi - base) / 2 ? (hi - base) / 2 : want));
+                            go = rest > kDonateMinTicks && hi - base > 2 * T;
+                        }
+                    }
+                    if (go) hi = donate_slices(w, base, hi, T);
                 }
             }
             int32_t i = base + ln;
@@ -877,6 +900,7 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
     const DevPlan& p = *w.pl;
     const int32_t stats = w.a.stats;
     if (donated) w.a.stats = 0;
+    const uint64_t t_build = w.a.queue ? (uint64_t)uni64((int64_t)wall_clock64()) : 0ull;
     int64_t l0 = w.a.row[v0], r0 = w.a.row[v0 + 1];
     for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0));
     int64_t l1 = w.a.row[v1], r1 = w.a.row[v1 + 1];
@@ -888,6 +912,7 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
         }
     }
     w.a.stats = stats;
+    if (w.a.queue) w.rebuild_ticks = uni64((int64_t)wall_clock64()) - (int64_t)t_build;
     w.emb[0] = v0;
     w.emb[1] = v1;
     w.slice_lo = lo;

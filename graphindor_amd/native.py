@@ -344,7 +344,12 @@ class Schedule:
         return list(buf[:n])
 
     def plan_desc(self, mode=GM_MODE_PATTERN):
-        return gs().gs_schedule_plan_desc(self.h, mode)
+        """Pointer to the flattened plan, which the schedule owns; the pointer
+        keeps the schedule alive, so a temporary schedule is safe to use."""
+        d = gs().gs_schedule_plan_desc(self.h, mode)
+        if d:
+            d._owner = self
+        return d
 
     def count_complete(self, m):
         return gs().gs_schedule_count_complete(self.h, m)
