@@ -211,8 +211,10 @@ __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
 }
 
 // Donation only pays for a remainder of at least this many wall-clock
-// ticks (10 ns each at the 100 MHz constant clock: 20 us).
-constexpr int64_t kDonateMinTicks = 2000;
+// ticks (10 ns each at the 100 MHz constant clock: 500 us).  Below that,
+// the queue traffic of many small donations costs more than the balance
+// gains (house and the 4-motifs on R-MAT: up to 50x slower at 20 us).
+constexpr int64_t kDonateMinTicks = 50000;
 
 // Time limit: true (and the wave marked aborted) once the wave has run
 // longer than the launch allows.  Uniform: every lane reads the same clock.
@@ -936,6 +938,7 @@ __device__ __forceinline__ int64_t pop_slice(const KernelArgs& a) {
                 break;
             }
             h = prev;
+            __builtin_amdgcn_s_sleep(2);  // contended: let the winner through
         }
     }
     return uni64(__shfl(got, 0));
