@@ -79,7 +79,9 @@ struct Counters {
     unsigned int overflow;
     unsigned int timed_out;        // a wave passed the launch's time limit
     unsigned long long big_cursor; // block-per-root bitmap kernel's vertex cursor
-    unsigned long long qhead;      // slice queue: next slot to pop
+    // The slice queue's counters on a cache line of their own, away from the
+    // root cursor every wave claims from.
+    alignas(128) unsigned long long qhead;  // slice queue: next slot to pop
     unsigned long long qtail;      // slice queue: slots reserved by pushers
     unsigned long long pending;    // slices pushed and not yet finished
 };
@@ -1023,7 +1025,9 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     while (true) {
         if (out_of_time(w)) break;
         // Donated slices first: they belong to roots already under way.
-        if (a.queue) {
+        // Lazy donation only pushes once the cursor is exhausted, so until
+        // then the queue is empty and is not polled.
+        if (a.queue && (main_done || a.eager_donation)) {
             const int64_t slot = pop_slice(a);
             if (slot >= 0) {
                 SliceItem* it = a.queue + slot;
