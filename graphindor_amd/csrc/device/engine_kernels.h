@@ -83,7 +83,11 @@ struct Counters {
     // root cursor every wave claims from.
     alignas(128) unsigned long long qhead;  // slice queue: next slot to pop
     unsigned long long qtail;      // slice queue: slots reserved by pushers
-    unsigned long long pending;    // slices pushed and not yet finished
+    unsigned long long done_items; // (root, plan) items fully processed, donations included
+    unsigned long long donations;  // donate_slices calls that queued slices
+    unsigned long long retries;    // queue-head CAS attempts lost to another popper
+    unsigned long long slice_ticks; // wall-clock ticks waves spent on donated slices (sum over waves)
+    unsigned long long idle_ticks;  // wall-clock ticks waves spent waiting for slices (sum over waves)
 };
 
 // A slice of one root edge's depth-2 loop set, donated by the wave that
@@ -201,6 +205,7 @@ struct Wave {
     uint64_t t_start;              // wall clock at the wave's start
     int32_t aborted;               // 1: the time limit passed; unwind
     int64_t rebuild_ticks;         // wall-clock ticks the current root's depth-0/1 builds took
+    int32_t donations;             // donations this wave made
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -217,6 +222,14 @@ __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
 // the queue traffic of many small donations costs more than the balance
 // gains (house and the 4-motifs on R-MAT: up to 50x slower at 20 us).
 constexpr int64_t kDonateMinTicks = 50000;
+
+// Wall-clock ticks a popper pays to claim a slice before its first candidate
+// (queue-head CAS under contention, the ready flag, the item's loads): a
+// slice is sized to carry at least 8x this plus the prefix rebuild.  Without
+// it, patterns whose prefixes cost nothing to rebuild (house, 4-motifs) were
+// cut into 8-candidate slices, 10^5 per launch, whose claims serialised on
+// the queue head.
+constexpr int64_t kSliceClaimTicks = 300;
 
 // Time limit: true (and the wave marked aborted) once the wave has run
 // longer than the launch allows.  Uniform: every lane reads the same clock.
@@ -760,7 +773,7 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
     const int64_t cap = w.a.queue_cap;
     const int32_t queued = base >= (unsigned long long)cap ? 0
                            : (int32_t)(((int64_t)base + nslices <= cap) ? nslices : cap - (int64_t)base);
-    if (queued > 0 && lane_id() == 0) atomicAdd(&w.a.ctr->pending, (unsigned long long)queued);
+    if (queued > 0) ++w.donations;
     // The last `queued` slices go to the queue; the ones before them that
     // did not fit stay with this wave, so its part remains contiguous.
     const int32_t first = nslices - queued;
@@ -772,9 +785,12 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
         it->g = w.cur_gr;
         it->s0 = keep_hi + k * T;
         it->s1 = keep_hi + (k + 1) * T < hi ? keep_hi + (k + 1) * T : hi;
-        __threadfence();
-        __hip_atomic_store(&it->ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // One release fence (an L2 write-back) for the whole donation orders
+    // every lane's item stores before its `ready` flags.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (int32_t q = lane_id(); q < queued; q += kWave)
+        __hip_atomic_store(&w.a.queue[base + q].ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int32_t kept = keep_hi + first * T;
     return kept < hi ? kept : hi;
 }
@@ -830,7 +846,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
                 // would keep this wave busy is cut into slices for idle waves.
                 // Slice length from this root's own costs: a slice's work
                 // (T candidates at the per-candidate time measured so far)
-                // is at least 8x the prefix rebuild a popper pays for it.
+                // is at least 8x the prefix rebuild and claim a popper pays for it.
                 if (w.donate && hi - base > 4 * w.a.slice_len) {
                     int32_t T = w.a.slice_len;
                     bool go = w.a.eager_donation != 0;
@@ -842,7 +858,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
                             const int64_t el = uni64((int64_t)wall_clock64()) - (int64_t)t_loop0;
                             const int64_t per = el / (base - lo) + 1;  // ticks per candidate
                             const int64_t rest = per * (int64_t)(hi - base);
-                            const int64_t want = (8 * w.rebuild_ticks) / per + 1;
+                            const int64_t want = (8 * (w.rebuild_ticks + kSliceClaimTicks)) / per + 1;
                             T = (int32_t)(want < T ? T : (want > (hi - base) / 2 ? (hi - base) / 2 : want));
                             go = rest > kDonateMinTicks && hi - base > 2 * T;
                         }
@@ -926,7 +942,7 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
 }
 
 // Claims the next donated slice (lane 0 CAS on the queue head); -1 if none.
-__device__ __forceinline__ int64_t pop_slice(const KernelArgs& a) {
+__device__ __forceinline__ int64_t pop_slice(const KernelArgs& a, unsigned long long& lost) {
     long long got = -1;
     if (lane_id() == 0) {
         unsigned long long h = __hip_atomic_load(&a.ctr->qhead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -940,6 +956,7 @@ __device__ __forceinline__ int64_t pop_slice(const KernelArgs& a) {
                 break;
             }
             h = prev;
+            ++lost;
             __builtin_amdgcn_s_sleep(2);  // contended: let the winner through
         }
     }
@@ -1022,18 +1039,72 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     int32_t cur_g = 0, cur_p = 0;
     bool main_done = false;
     int idle_sleeps = 1;
+    int64_t ticket = -1;  // queue slot this idle wave holds (wave-uniform)
+    // Queue diagnostics (lane 0's CAS losses; wave-uniform tick sums).
+    unsigned long long lost = 0, slice_ticks = 0, idle_ticks = 0;
+    w.donations = 0;
     while (true) {
         if (out_of_time(w)) break;
         // Donated slices first: they belong to roots already under way.
         // Lazy donation only pushes once the cursor is exhausted, so until
         // then the queue is empty and is not polled.
         if (a.queue && (main_done || a.eager_donation)) {
-            const int64_t slot = pop_slice(a);
+            const uint64_t t_pop = (uint64_t)uni64((int64_t)wall_clock64());
+            int64_t slot = -1;
+            if (main_done) {
+                // An idle wave holds a ticket: the next queue slot, taken
+                // with one fetch-add (a CAS claim costs one attempt per
+                // contending wave, and thousands of waves idle at the end).
+                // It waits until its slot is pushed, or leaves once every
+                // root item is done -- only roots push, so the tail is then
+                // final -- and the ticket lies past the tail.  (Counting
+                // items, not waves, keeps the rule independent of how many
+                // of the grid's waves are resident.)
+                if (ticket < 0) {
+                    unsigned long long t = 0;
+                    if (ln == 0) t = atomicAdd(&a.ctr->qhead, 1ull);
+                    ticket = uni64(__shfl((long long)t, 0));
+                }
+                if (ticket >= a.queue_cap) break;  // the queue is used up: no slot will come
+                unsigned long long tail = 0;
+                if (ln == 0) tail = __hip_atomic_load(&a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tail = (unsigned long long)uni64(__shfl((long long)tail, 0));
+                if ((unsigned long long)ticket < tail) {
+                    slot = ticket;
+                    ticket = -1;
+                } else {
+                    unsigned long long fin = 0;
+                    if (ln == 0)
+                        fin = __hip_atomic_load(&a.ctr->done_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    fin = (unsigned long long)uni64(__shfl((long long)fin, 0));
+                    if ((int64_t)fin >= nitems) {
+                        // Pairs with the release adds of done_items: every
+                        // push reserved its slots before its root's add.
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                        if (ln == 0) tail = __hip_atomic_load(&a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        tail = (unsigned long long)uni64(__shfl((long long)tail, 0));
+                        if ((unsigned long long)ticket >= tail) break;
+                        continue;  // pushed meanwhile: the next pass takes it
+                    }
+                    // Back off exponentially; relaxed polls, nothing is read
+                    // on their strength (an acquire would invalidate this
+                    // XCD's L2 under the waves still walking adjacency lists).
+                    for (int k = 0; k < idle_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
+                    idle_sleeps = idle_sleeps < 16 ? idle_sleeps * 2 : 16;
+                    idle_ticks += uni64((int64_t)wall_clock64()) - (int64_t)t_pop;
+                    continue;
+                }
+            } else {
+                slot = pop_slice(a, lost);  // still holds roots: never wait
+            }
             if (slot >= 0) {
                 SliceItem* it = a.queue + slot;
+                // The slot is reserved, so its pusher is writing it: spin
+                // with relaxed loads, then acquire once.
                 if (ln == 0)
-                    while (__hip_atomic_load(&it->ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    while (__hip_atomic_load(&it->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
                         __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 wave_sync();
                 const int64_t e = uni64(__shfl((long long)__hip_atomic_load(&it->e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0));
                 const int32_t pj = unii(__shfl(it->pl, 0));
@@ -1051,22 +1122,12 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 const uint32_t u1 = a.col[e];
                 const uint32_t u0 = (uint32_t)edge_source(a.row, a.nv, e);
                 process_root<MODE, NMAX>(w, u0, u1, s0, s1, true);
-                if (ln == 0) atomicAdd(&a.ctr->pending, ~0ull);  // one slice done
+                slice_ticks += uni64((int64_t)wall_clock64()) - (int64_t)t_pop;
                 idle_sleeps = 1;
                 continue;
             }
         }
-        if (main_done) {
-            // Nothing to claim; stop once no donated slice is outstanding.
-            // Idle waves back off exponentially so their polling of the
-            // queue counters does not crowd the L2 channel those live on.
-            const unsigned long long pend =
-                a.queue ? __hip_atomic_load(&a.ctr->pending, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
-            if (uni64((int64_t)__shfl((long long)pend, 0)) == 0) break;
-            for (int k = 0; k < idle_sleeps; ++k) __builtin_amdgcn_s_sleep(127);
-            idle_sleeps = idle_sleeps < 16 ? idle_sleeps * 2 : 16;
-            continue;
-        }
+        if (main_done) break;
         idle_sleeps = 1;
         unsigned long long start = 0;
         int64_t take = 0;
@@ -1124,6 +1185,10 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 process_root<MODE, NMAX>(w, u0, u1, 0, 0x7fffffff, false);
             }
         }
+        // Release: the pushes (tail reservations) of these roots precede it.
+        if (a.queue && ln == 0 && !w.aborted)
+            __hip_atomic_fetch_add(&a.ctr->done_items, (unsigned long long)(i_end - (int64_t)start), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
     }
     flush_sum(w, cur_g, cur_p);
     const unsigned long long lane_bytes = a.stats ? wave_sum(w.lane_bytes) : 0ull;
@@ -1132,6 +1197,12 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
         if (a.stats) atomicAdd(&a.ctr->bytes, w.bytes + lane_bytes);
         if (w.overflow) atomicOr(&a.ctr->overflow, 1u);
         if (w.aborted) atomicOr(&a.ctr->timed_out, 1u);
+        if (a.queue) {
+            if (w.donations) atomicAdd(&a.ctr->donations, (unsigned long long)w.donations);
+            if (lost) atomicAdd(&a.ctr->retries, lost);
+            if (slice_ticks) atomicAdd(&a.ctr->slice_ticks, slice_ticks);
+            if (idle_ticks) atomicAdd(&a.ctr->idle_ticks, idle_ticks);
+        }
     }
 }
 

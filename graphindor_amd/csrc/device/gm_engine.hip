@@ -122,15 +122,28 @@ struct DeviceCache {
     SliceItem* queue = nullptr;      // donated depth-2 slices (ready flags all-clear between launches)
 };
 
-constexpr int64_t kQueueCap = 1 << 18;
+constexpr int64_t kQueueCap = 1 << 20;  // 32 MiB of slice items per device
 
-// Slice length from the tuning: > 0 donates in slices of that length,
-// 0 (default) and -1 do not donate.  Measured on R-MAT 2^14..2^18, donation
-// at depth 2 costs more than it saves (a slice rebuilds the root's
-// intersections, while the imbalance of the 5-cycle sits a level deeper),
-// so it is opt-in.
+// The slice queue's counters of a list-engine launch (wall clock: 100 MHz).
+void fill_queue_stats(const Counters& c, gm_stats* st) {
+    st->slices = c.qtail < (unsigned long long)kQueueCap ? c.qtail : (unsigned long long)kQueueCap;
+    st->donations = c.donations;
+    st->queue_retries = c.retries;
+    st->slice_wave_ms = (double)c.slice_ticks * 1e-5;
+    st->idle_wave_ms = (double)c.idle_ticks * 1e-5;
+}
+
+// Slice length from the tuning: > 0 donates in slices of at least that
+// length, 0 selects the default, -1 does not donate.  On R-MAT 2^14 and 2^16
+// (root-sampled, lane tail off) donation took house from 36 to 19 ms and
+// 123 to 48 ms, k4+pendant from 7.9 to 4.4 ms and 20.5 to 5.7 ms, the
+// 5-cycle from 1.42 to 0.65 s and 10.0 to 3.3 s; with the lane tail on,
+// house and k4+pendant donate nothing (their depth 2 is the tail) and the
+// 5-cycle gains 2.5x.
+constexpr int32_t kDefaultSliceLen = 8;
 int32_t slice_len_of(const gm_tuning* t) {
-    return (t && t->slice_len > 0) ? t->slice_len : 0;
+    if (t && t->slice_len < 0) return 0;
+    return (t && t->slice_len > 0) ? t->slice_len : kDefaultSliceLen;
 }
 
 hipError_t alloc_queue(SliceItem** q) {
@@ -739,6 +752,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         stats->bitmap_alg_bytes = cb.bytes;
         stats->bitmap_block_ms = block_ms;
         stats->bitmap_block_alg_bytes = ck.bytes;
+        fill_queue_stats(c, stats);
     }
     return GM_OK;
 }
@@ -913,6 +927,7 @@ int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats)
         stats->bitmap_alg_bytes = 0;
         stats->bitmap_block_ms = 0;
         stats->bitmap_block_alg_bytes = 0;
+        fill_queue_stats(c, stats);
     }
     return GM_OK;
 }

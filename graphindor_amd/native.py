@@ -63,7 +63,9 @@ class GmStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("alg_bytes", ctypes.c_uint64), ("root_tasks", ctypes.c_uint64),
                 ("grid_waves", ctypes.c_int32), ("flags_used", ctypes.c_int32), ("list_ms", ctypes.c_double),
                 ("bitmap_ms", ctypes.c_double), ("bitmap_alg_bytes", ctypes.c_uint64),
-                ("bitmap_block_ms", ctypes.c_double), ("bitmap_block_alg_bytes", ctypes.c_uint64)]
+                ("bitmap_block_ms", ctypes.c_double), ("bitmap_block_alg_bytes", ctypes.c_uint64),
+                ("slices", ctypes.c_uint64), ("donations", ctypes.c_uint64), ("queue_retries", ctypes.c_uint64),
+                ("slice_wave_ms", ctypes.c_double), ("idle_wave_ms", ctypes.c_double)]
 
 
 _gm = None

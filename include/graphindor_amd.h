@@ -105,7 +105,7 @@ typedef struct gm_tuning {
     int32_t slice_len;         /* GM_MODE_PATTERN: once no root is left to claim, a wave whose current
                                   root still has more than 4*slice_len depth-2 candidates keeps
                                   slice_len of them and donates the rest, in slices of slice_len, to
-                                  idle waves (0 or -1 = off, the default) */
+                                  idle waves, which claim them from a queue (0 = default 8, -1 = off) */
     int32_t bitmap_block_degree; /* GM_MODE_CLIQUE: bitmap roots of out-degree above this whose rows fit
                                   in one block's LDS are counted by a whole block each (0 = default
                                   128, -1 = off: every bitmap root by one wave) */
@@ -126,6 +126,12 @@ typedef struct gm_stats {
     uint64_t bitmap_alg_bytes;/* the bitmap engine's share of alg_bytes */
     double bitmap_block_ms;   /* of bitmap_ms: the block-per-root launches (large roots) */
     uint64_t bitmap_block_alg_bytes; /* of bitmap_alg_bytes: the block-per-root launches */
+    /* Depth-2 slice donation (tuning.slice_len > 0), list engine only: */
+    uint64_t slices;          /* slices pushed to the queue */
+    uint64_t donations;       /* donations (each pushes one or more slices) */
+    uint64_t queue_retries;   /* queue-head claims lost to another wave */
+    double slice_wave_ms;     /* wall time waves spent on donated slices, summed over waves */
+    double idle_wave_ms;      /* wall time waves spent waiting for slices, summed over waves */
 } gm_stats;
 
 /* gm_count flags */

@@ -470,7 +470,7 @@ def test_slice_donation_keeps_sums_and_bytes(name):
     assert ref[0] == (oraw, ob)
     assert ref[-1][0] == oraw
     off = oraw
-    for sl in (1, 4, 32):
+    for sl in (0, 1, 4, 32):  # 0: the default length
         for tail in (-1, 0):
             print(f"slice_len {sl} lane_tail {tail}", flush=True)
             for eager in (0, N.GM_COUNT_EAGER_DONATION):
