@@ -159,7 +159,10 @@ def pm_leg(N, scale=22, seed=4, budget_s=8.0):
     model is present, the learned pick.  A full count on R-MAT 2^22 is far
     beyond a bench run (hub neighbourhoods of 10^5 vertices), so each pattern
     is timed on the root sample e % S == 0, S halving from 2^16 until the
-    analytic run takes about budget_s / 8; both schedules use the same S.
+    analytic run takes about budget_s / 8 (or doubling up to 2^20 while it
+    exceeds budget_s); both schedules use the same S.  The analytic schedule
+    is also timed with the lane-parallel tail off (`analytic_tail_off_ms`):
+    with depth-2 slice donation that is faster on several patterns.
     Sampled sums differ between schedules (a root edge maps to different
     embeddings), so the comparison is of times; full-count parity is in the
     GPU tests."""
@@ -181,13 +184,18 @@ def pm_leg(N, scale=22, seed=4, budget_s=8.0):
         limit = int(budget_s * 1000)
         S = 1 << 16
 
-        def timed(plan):
+        def timed(plan, **tuning):
             try:
-                return N.count_batch(dg, [0, len(e)], [plan], root_stride=S, time_limit_ms=limit)[1].kernel_ms
+                return N.count_batch(dg, [0, len(e)], [plan], root_stride=S, time_limit_ms=limit,
+                                     **tuning)[1].kernel_ms
             except N.GmTimeout:
                 return None
         t = timed(pa)
-        while t is not None and t / 1e3 < budget_s / 8 and S > 1:
+        while t is None and S < (1 << 20):
+            S *= 4
+            t = timed(pa)
+        grown = S > (1 << 16)
+        while not grown and t is not None and t / 1e3 < budget_s / 8 and S > 1:
             S //= 2
             t = timed(pa)
         row.update({"root_stride": S, "analytic_candidate": ia["analytic"], "candidates": ia["candidates"],
@@ -200,6 +208,8 @@ def pm_leg(N, scale=22, seed=4, budget_s=8.0):
             continue
         t = timed(pa)
         row["analytic_ms"] = round(t, 3) if t is not None else f"over {limit} ms"
+        t_off = timed(pa, lane_tail=-1)
+        row["analytic_tail_off_ms"] = round(t_off, 3) if t_off is not None else f"over {limit} ms"
         if model is not None and t is not None:
             sl, il = N.select_schedule_measured(5, adj, g, model)
             t2 = timed(N.DevicePlan(sl.plan_desc()))

@@ -637,6 +637,8 @@ __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, 
 
 template <int D>
 __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound);
+__device__ __forceinline__ int32_t maybe_donate(Wave& w, int32_t lo, int32_t base, int32_t hi, uint64_t t_loop0,
+                                                int32_t step_mask);
 
 template <int D>
 __device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
@@ -718,10 +720,20 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     const int ln = lane_id();
     const TailCtx& c = w.lds->tail;
     unsigned long long acc = 0, bytes = 0;
-    for (int32_t b0 = 0; b0 < loop.len; b0 += kWave) {
+    // At depth 2 a root's candidates may be split and donated exactly as in
+    // the DFS loop: every candidate's contribution is its own.
+    int32_t lo = 0, hi = loop.len;
+    uint64_t t_loop0 = 0;
+    if constexpr (D == 2) {
+        lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
+        hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
+        if (w.donate) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
+    }
+    for (int32_t b0 = lo; b0 < hi; b0 += kWave) {
         if (out_of_time(w)) break;
+        if constexpr (D == 2) hi = maybe_donate(w, lo, b0, hi, t_loop0, 15);
         const int32_t i = b0 + ln;
-        const bool in_range = i < loop.len;
+        const bool in_range = i < hi;
         const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
         const bool live = in_range && v < bound;
         const bool stop_after = ballot(in_range && v >= bound) != 0;
@@ -795,6 +807,36 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
     return kept < hi ? kept : hi;
 }
 
+// Donation check of a depth-2 loop at candidate `base` of [lo, hi): once no
+// root is left to claim, a remainder that would keep this wave busy is cut
+// into slices for idle waves; returns the new end of this wave's part.
+// Slice length from this root's own costs: a slice's work (T candidates at
+// the per-candidate time measured since t_loop0) is at least 8x the prefix
+// rebuild and claim a popper pays for it.  The check runs at the 64-candidate
+// steps whose index has no bit of step_mask set (the lane tail walks a step in
+// about a microsecond, so it checks every 16th).  Wave-uniform.
+__device__ __forceinline__ int32_t maybe_donate(Wave& w, int32_t lo, int32_t base, int32_t hi, uint64_t t_loop0,
+                                                int32_t step_mask) {
+    if (!w.donate || hi - base <= 4 * w.a.slice_len) return hi;
+    if ((((base - lo) / kWave) & step_mask) != 0) return hi;
+    int32_t T = w.a.slice_len;
+    bool go = w.a.eager_donation != 0;
+    if (!go && base > lo) {
+        unsigned long long c = 0;
+        if (lane_id() == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c = (unsigned long long)uni64(__shfl((long long)c, 0));
+        if ((int64_t)c >= w.nitems) {
+            const int64_t el = uni64((int64_t)wall_clock64()) - (int64_t)t_loop0;
+            const int64_t per = el / (base - lo) + 1;  // ticks per candidate
+            const int64_t rest = per * (int64_t)(hi - base);
+            const int64_t want = (8 * (w.rebuild_ticks + kSliceClaimTicks)) / per + 1;
+            T = (int32_t)(want < T ? T : (want > (hi - base) / 2 ? (hi - base) / 2 : want));
+            go = rest > kDonateMinTicks && hi - base > 2 * T;
+        }
+    }
+    return go ? donate_slices(w, base, hi, T) : hi;
+}
+
 template <int D, int MODE, int NMAX>
 __device__ __forceinline__ void dfs(Wave& w) {
     if constexpr (D >= NMAX) {
@@ -841,31 +883,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
             if (w.donate) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
         }
         for (int32_t base = lo; base < hi; base += kWave) {
-            if constexpr (D == 2) {
-                // Donation: once no root is left to claim, a remainder that
-                // would keep this wave busy is cut into slices for idle waves.
-                // Slice length from this root's own costs: a slice's work
-                // (T candidates at the per-candidate time measured so far)
-                // is at least 8x the prefix rebuild and claim a popper pays for it.
-                if (w.donate && hi - base > 4 * w.a.slice_len) {
-                    int32_t T = w.a.slice_len;
-                    bool go = w.a.eager_donation != 0;
-                    if (!go && base > lo) {
-                        unsigned long long c = 0;
-                        if (ln == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        c = (unsigned long long)uni64(__shfl((long long)c, 0));
-                        if ((int64_t)c >= w.nitems) {
-                            const int64_t el = uni64((int64_t)wall_clock64()) - (int64_t)t_loop0;
-                            const int64_t per = el / (base - lo) + 1;  // ticks per candidate
-                            const int64_t rest = per * (int64_t)(hi - base);
-                            const int64_t want = (8 * (w.rebuild_ticks + kSliceClaimTicks)) / per + 1;
-                            T = (int32_t)(want < T ? T : (want > (hi - base) / 2 ? (hi - base) / 2 : want));
-                            go = rest > kDonateMinTicks && hi - base > 2 * T;
-                        }
-                    }
-                    if (go) hi = donate_slices(w, base, hi, T);
-                }
-            }
+            if constexpr (D == 2) hi = maybe_donate(w, lo, base, hi, t_loop0, 0);
             int32_t i = base + ln;
             uint32_t cand = 0xffffffffu;
             int64_t rb = 0, re = 0;

@@ -12,7 +12,7 @@ import pytest
 
 from graphindor_amd import native as N
 from oracle import oracle as O
-from tests.common import (MOTIFS4, PATTERNS, clique_adj, complete_graph, copies_in_complete, hub_graph,
+from tests.common import (MOTIFS4, PATTERNS, clique_adj, complete_graph, copies_in_complete, csr_from_adjsets, hub_graph,
                           random_graph, reduce_for_clique, require_gpu)
 
 pytestmark = pytest.mark.gpu
@@ -157,6 +157,50 @@ def test_hub_larger_than_a_chunk():
         s = N.Schedule.plain(k, clique_adj(k))
         s.redundancy = 1
         gpu_vs_oracle(rv, re_, s, N.GM_MODE_CLIQUE)
+
+
+def test_tail_slices_of_hub_roots_keep_counts_and_bytes():
+    """The lane tail at depth 2 (house) splits and donates a hub root's
+    candidates in slices, like the DFS loop: the oracle's sums and byte
+    tallies hold with donation off, lazy (default) and eager, and the eager
+    runs did push slices."""
+    # Two adjacent hubs with 5000 common neighbours (so N(h0) and N(h0) ∩ N(h1)
+    # are both long), plus sparse leaf-leaf edges; the hubs carry the
+    # highest ids in one labelling and the lowest in the other, since the
+    # depth-1 restriction v1 < v0 decides which roots start at a hub.
+    rng = np.random.default_rng(7)
+    nl = 5000
+    nv = nl + 2
+    adj = [set() for _ in range(nv)]
+    for h in (nl, nl + 1):
+        for x in range(nl):
+            adj[h].add(x)
+            adj[x].add(h)
+    adj[nl].add(nl + 1)
+    adj[nl + 1].add(nl)
+    for a, b in rng.integers(0, nl, size=(6000, 2)):
+        if a != b:
+            adj[a].add(int(b))
+            adj[b].add(int(a))
+    adj_r = [set(nv - 1 - x for x in adj[nv - 1 - a]) for a in range(nv)]
+    pushed = False
+    for v, e in (csr_from_adjsets(adj), csr_from_adjsets(adj_r)):
+        csr = O.CSR(v, e)
+        dg = N.DeviceGraph(v, e)
+        for name in ("house", "tailed_triangle"):
+            n, padj = PATTERNS[name]
+            s = N.Schedule.build(n, padj, nv, len(e), 0)
+            desc = s.plan_desc()
+            dp = N.DevicePlan(desc)
+            oraw, ob = O.count_edge_rooted(csr, desc)
+            for sl, eager in ((-1, 0), (0, 0), (0, N.GM_COUNT_EAGER_DONATION), (1, N.GM_COUNT_EAGER_DONATION)):
+                print(f"case: two hubs of 5000 {name}, hubs first {v[1] - v[0] > nl}, slice_len {sl}, eager {eager}",
+                      flush=True)
+                raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES | eager, slice_len=sl)
+                assert (raw, st.alg_bytes) == (oraw, ob), (name, sl, eager)
+                if eager and name == "house":
+                    pushed |= st.slices > 0
+    assert pushed
 
 
 @pytest.mark.parametrize("waves", [1, 4, 16])
