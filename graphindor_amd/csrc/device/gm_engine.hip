@@ -183,7 +183,7 @@ int bm_blocks(int cus) { return cus * 10; }
 
 // Block-per-root kernel: rows of d x W words in dynamic LDS, up to what a
 // block may allocate beside the kernel's static LDS.
-constexpr int kBigMinD = 128;
+constexpr int kBigMinD = 64;
 
 hipError_t big_rows_capacity(int device, int32_t* cap_words, size_t* dyn_bytes) {
     int maxsh = 0;

@@ -108,7 +108,7 @@ typedef struct gm_tuning {
                                   idle waves, which claim them from a queue (0 = default 8, -1 = off) */
     int32_t bitmap_block_degree; /* GM_MODE_CLIQUE: bitmap roots of out-degree above this whose rows fit
                                   in one block's LDS are counted by a whole block each (0 = default
-                                  128, -1 = off: every bitmap root by one wave) */
+                                  64, -1 = off: every bitmap root by one wave) */
     int32_t time_limit_ms;     /* GM_MODE_PATTERN / list engine: waves stop once they have run this long
                                   and the call fails with GM_ERR_TIMEOUT (0 = no limit; for timing
                                   harnesses that must not wait on a slow candidate) */
