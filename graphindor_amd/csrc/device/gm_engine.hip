@@ -198,6 +198,23 @@ hipError_t big_rows_capacity(int device, int32_t* cap_words, size_t* dyn_bytes) 
     return hipSuccess;
 }
 
+// Resident blocks of the block-per-root kernel with `dyn` bytes of row LDS:
+// one launch fills every slot (the kernel claims roots from a cursor).
+int big_blocks(int k, int cus, size_t dyn) {
+    int occ = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (k) {
+        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<3>, kBigThreads, dyn); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<4>, kBigThreads, dyn); break;
+        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<5>, kBigThreads, dyn); break;
+        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<6>, kBigThreads, dyn); break;
+        case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<7>, kBigThreads, dyn); break;
+        default: break;
+    }
+    if (e != hipSuccess || occ < 1) occ = 1;
+    return cus * std::min(occ, 4);
+}
+
 hipError_t launch_bitmap_big(int k, int blocks, size_t dyn, const BmArgs& b, hipStream_t s) {
     switch (k) {
         case 3: hipLaunchKernelGGL((clique_bitmap_big_kernel<3>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
@@ -659,7 +676,18 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             size_t big_dyn = 0;
             b.big_min_d = (t && t->bitmap_block_degree > 0) ? t->bitmap_block_degree : kBigMinD;
             b.big_cap_words = 0;
-            if (!(t && t->bitmap_block_degree < 0)) HIP_TRY(big_rows_capacity(g->device, &b.big_cap_words, &big_dyn));
+            if (!(t && t->bitmap_block_degree < 0)) {
+                HIP_TRY(big_rows_capacity(g->device, &b.big_cap_words, &big_dyn));
+                // Rows only as large as this graph's longest bitmap root
+                // needs, so that more blocks are resident per CU (a root
+                // whose rows exceed the capacity goes to the per-wave kernel).
+                const int32_t dmax = std::min<int32_t>(g->max_deg, bm_max);
+                const int32_t need = dmax * ((dmax + 63) / 64);
+                if (need > 0 && need < b.big_cap_words) {
+                    b.big_cap_words = need;
+                    big_dyn = (size_t)need * sizeof(unsigned long long);
+                }
+            }
             hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_bm,
                                (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
             HIP_TRY(hipEventRecord(dc.evpool[evb], s));
@@ -671,7 +699,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
                 BmArgs bb = b;
                 bb.ctr = ctr_blk;
                 HIP_TRY(hipEventRecord(dc.evpool[evk], s));
-                HIP_TRY(launch_bitmap_big(p->host.n, cus, big_dyn, bb, s));
+                HIP_TRY(launch_bitmap_big(p->host.n, big_blocks(p->host.n, cus, big_dyn), big_dyn, bb, s));
                 HIP_TRY(hipEventRecord(dc.evpool[evk + 1], s));
                 ev_block.push_back(evk);
                 hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm, ctr_blk);
