@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # 2^20 ef16 seed 2, degree-oriented), from two rocprofv3 PMC runs (FETCH_SIZE
 # and WRITE_SIZE, each scaled by what it reads for a 1 GiB device copy in the
 # same run); see profiles/clique5_rmat20_hbm_traffic.md.
-TRAFFIC = {(20, "clique_bitmap_big_kernel<5>"): 9.176e9}
+TRAFFIC = {(20, "clique_bitmap_big_kernel<5>"): 9.191e9}
 
 
 def log(*a):

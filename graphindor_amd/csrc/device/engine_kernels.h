@@ -1481,7 +1481,7 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
 // bm_root.
 // ---------------------------------------------------------------------------
 constexpr int kBigThreads = 512;
-constexpr int kBigChunk = 64;  // vertices per cursor claim (fine enough that the last claims balance)
+constexpr int kBigChunk = 16;  // vertices per cursor claim (fine enough that the last claims balance)
 
 template <int K>
 __global__ void __launch_bounds__(kBigThreads) clique_bitmap_big_kernel(BmArgs a) {
