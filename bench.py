@@ -258,7 +258,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the counting path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the counting path runs on one GPU; several would each take a range of the root edges)", file=sys.stderr)
         sys.exit(2)
 
     from graphindor_amd import native as N

@@ -959,7 +959,9 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
     dfs<2, MODE, NMAX>(w);
 }
 
-// Claims the next donated slice (lane 0 CAS on the queue head); -1 if none.
+// Claims the next pushed slice without waiting (lane 0 CAS on the queue head;
+// -1 if none): the claim of a wave that still holds roots, under
+// GM_COUNT_EAGER_DONATION.  Idle waves take tickets instead.
 __device__ __forceinline__ int64_t pop_slice(const KernelArgs& a, unsigned long long& lost) {
     long long got = -1;
     if (lane_id() == 0) {
