@@ -4,6 +4,7 @@ device call)."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -83,3 +84,21 @@ def test_bad_plans_are_rejected():
 def test_count_rejects_null_handles():
     raw = ctypes.c_uint64()
     assert N.gm().gm_count(None, None, None, 0, None, ctypes.byref(raw), None) == -1
+
+
+@pytest.mark.parametrize("ctype,cname", [(N.GmStats, "gm_stats"), (N.GmTuning, "gm_tuning")])
+def test_ctypes_mirrors_match_the_header_layout(tmp_path, ctype, cname):
+    """The binding's structs have the header's size and field offsets (a C
+    program built against include/graphindor_amd.h prints them)."""
+    names = [f[0] for f in ctype._fields_]
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "graphindor_amd.h"', "int main(void) {",
+             f'    printf("%zu\\n", sizeof({cname}));']
+    lines += [f'    printf("%zu\\n", offsetof({cname}, {n}));' for n in names]
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(ctype)
+    assert out[1:] == [getattr(ctype, n).offset for n in names]
