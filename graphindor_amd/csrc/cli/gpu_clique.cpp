@@ -1,9 +1,10 @@
 // gpu_clique <graph.g> <clique_size>
 // Same arguments and result lines as the reference (gpu/gpu_clique.cu:573-640):
 // "count %llu" and "Counting time cost: S.UUUUUU s.".  Edges are oriented by
-// (degree, id) (degree_orientation_init) unless
-// GRAPHSET_CLIQUE_ORIENTATION=id asks for the reference's reduce_edges_for_clique;
-// the count is the same either way.
+// (degree, id) (degree_orientation_init) unless GRAPHSET_CLIQUE_ORIENTATION
+// asks for the reference's reduce_edges_for_clique ("id") or for the same
+// orientation with the vertices renumbered in (degree, id) order ("rank",
+// degree_rank_orientation); the count is the same in every case.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -44,6 +45,8 @@ int main(int argc, char* argv[]) {
     if (orient && strcmp(orient, "id") == 0) {
         reduce_edges_for_clique(*g);
         dag = g;
+    } else if (orient && strcmp(orient, "rank") == 0) {
+        degree_rank_orientation(g, dag);
     } else {
         degree_orientation_init(g, dag);
     }

@@ -40,6 +40,7 @@ __global__ void accumulate_counters_kernel(Counters* total, const Counters* part
     total->bytes += part->bytes;
     total->tasks += part->tasks;
     total->overflow |= part->overflow;
+    total->timed_out |= part->timed_out;
 }
 
 // Zeroes n 64-bit words (a kernel, so that it may be captured in a graph).
@@ -200,31 +201,60 @@ hipError_t big_rows_capacity(int device, int32_t* cap_words, size_t* dyn_bytes) 
 
 // Resident blocks of the block-per-root kernel with `dyn` bytes of row LDS:
 // one launch fills every slot (the kernel claims roots from a cursor).
-int big_blocks(int k, int cus, size_t dyn) {
-    int occ = 0;
-    hipError_t e = hipErrorInvalidValue;
+// The block kernel for clique size k and row triangularity tri (used for k = 4, 5).
+template <int K, int TRI>
+void* big_kernel_fn() { return reinterpret_cast<void*>(&clique_bitmap_big_kernel<K, TRI>); }
+
+void* big_kernel_for(int k, int tri) {
     switch (k) {
-        case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<3>, kBigThreads, dyn); break;
-        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<4>, kBigThreads, dyn); break;
-        case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<5>, kBigThreads, dyn); break;
-        case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<6>, kBigThreads, dyn); break;
-        case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, clique_bitmap_big_kernel<7>, kBigThreads, dyn); break;
-        default: break;
+        case 3: return big_kernel_fn<3, 0>();
+        case 4: return tri > 0 ? big_kernel_fn<4, 1>() : tri < 0 ? big_kernel_fn<4, -1>() : big_kernel_fn<4, 0>();
+        case 5: return tri > 0 ? big_kernel_fn<5, 1>() : tri < 0 ? big_kernel_fn<5, -1>() : big_kernel_fn<5, 0>();
+        case 6: return big_kernel_fn<6, 0>();
+        case 7: return big_kernel_fn<7, 0>();
+        default: return nullptr;
     }
+}
+
+int big_blocks(int k, int tri, int cus, size_t dyn) {
+    int occ = 0;
+    void* fn = big_kernel_for(k, tri);
+    hipError_t e = fn ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kBigThreads, dyn) : hipErrorInvalidValue;
     if (e != hipSuccess || occ < 1) occ = 1;
     return cus * std::min(occ, 4);
 }
 
-hipError_t launch_bitmap_big(int k, int blocks, size_t dyn, const BmArgs& b, hipStream_t s) {
+hipError_t launch_bitmap_big(int k, int tri, int blocks, size_t dyn, const BmArgs& b, hipStream_t s) {
+    void* fn = big_kernel_for(k, tri);
+    if (!fn) return hipErrorInvalidValue;
+    BmArgs arg = b;
+    void* params[] = {&arg};
+    return hipLaunchKernel(fn, dim3(blocks), dim3(kBigThreads), params, dyn, s);
+}
+
+// Small-root kernel: one 64-lane block per resident slot.
+template <int K>
+void* small_kernel_fn() { return reinterpret_cast<void*>(&clique_bitmap_small_kernel<K>); }
+
+void* small_kernel_for(int k) {
     switch (k) {
-        case 3: hipLaunchKernelGGL((clique_bitmap_big_kernel<3>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
-        case 4: hipLaunchKernelGGL((clique_bitmap_big_kernel<4>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
-        case 5: hipLaunchKernelGGL((clique_bitmap_big_kernel<5>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
-        case 6: hipLaunchKernelGGL((clique_bitmap_big_kernel<6>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
-        case 7: hipLaunchKernelGGL((clique_bitmap_big_kernel<7>), dim3(blocks), dim3(kBigThreads), dyn, s, b); break;
-        default: return hipErrorInvalidValue;
+        case 3: return small_kernel_fn<3>();
+        case 4: return small_kernel_fn<4>();
+        case 5: return small_kernel_fn<5>();
+        case 6: return small_kernel_fn<6>();
+        case 7: return small_kernel_fn<7>();
+        default: return nullptr;
     }
-    return hipGetLastError();
+}
+
+hipError_t launch_bitmap_small(int k, int cus, const BmArgs& b, hipStream_t s) {
+    void* fn = small_kernel_for(k);
+    if (!fn) return hipErrorInvalidValue;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kWave, 0) != hipSuccess || occ < 1) occ = 8;
+    BmArgs arg = b;
+    void* params[] = {&arg};
+    return hipLaunchKernel(fn, dim3(cus * std::min(occ, 32)), dim3(kWave), params, 0, s);
 }
 
 hipError_t launch_bitmap(int k, int blocks, const BmArgs& b, hipStream_t s) {
@@ -249,6 +279,7 @@ struct gm_graph {
     uint32_t* d_col;
     int32_t max_deg;
     int32_t second_deg;
+    int32_t tri;  // 1: every row holds larger ids only; -1: smaller ids only; 0: neither (or no edges)
 };
 
 struct gm_batch;
@@ -279,6 +310,7 @@ int gm_graph_create(const int64_t* row_ptr, const int32_t* col, int32_t n, int64
     // Validate: monotone rows, ids in range, strictly increasing within a row
     // (sorted, no duplicates) and no self-loops.  Kernel indexing relies on it.
     int64_t d1 = 0, d2 = 0;
+    bool upper = nnz > 0, lower = nnz > 0;
     for (int32_t v = 0; v < n; ++v) {
         int64_t l = row_ptr[v], r = row_ptr[v + 1];
         if (r < l) return fail(GM_ERR_INVALID, "gm_graph_create: row_ptr not monotone at row " + std::to_string(v));
@@ -287,6 +319,11 @@ int gm_graph_create(const int64_t* row_ptr, const int32_t* col, int32_t n, int64
             if (u < 0 || u >= n) return fail(GM_ERR_INVALID, "gm_graph_create: neighbour id out of range in row " + std::to_string(v));
             if (u == v) return fail(GM_ERR_INVALID, "gm_graph_create: self-loop at vertex " + std::to_string(v));
             if (e > l && col[e - 1] >= u) return fail(GM_ERR_INVALID, "gm_graph_create: row " + std::to_string(v) + " not strictly increasing");
+        }
+        // Rows are sorted: their first and last ids decide the orientation.
+        if (r > l) {
+            upper = upper && col[l] > v;
+            lower = lower && col[r - 1] < v;
         }
         int64_t d = r - l;
         if (d > d1) { d2 = d1; d1 = d; }
@@ -300,6 +337,7 @@ int gm_graph_create(const int64_t* row_ptr, const int32_t* col, int32_t n, int64
     g->nnz = nnz;
     g->max_deg = (int32_t)d1;
     g->second_deg = (int32_t)d2;
+    g->tri = upper ? 1 : (lower ? -1 : 0);
     g->d_row = nullptr;
     g->d_col = nullptr;
     hipError_t e1 = hipMalloc(&g->d_row, sizeof(int64_t) * ((size_t)n + 1));
@@ -340,7 +378,7 @@ int gm_plan_create(const gm_plan_desc* d, gm_plan** out) {
         return fail(GM_ERR_INVALID, "gm_plan_create: bad prefix counts");
     const int k = d->in_exclusion_optimize_num;
     if (k < 0 || k > n - 2 || k == 1) return fail(GM_ERR_INVALID, "gm_plan_create: bad IEP depth");
-    if (d->iep_vertex_num < 0 || d->iep_vertex_num > GM_MAX_IEP || d->iep_array_size < 0 || d->iep_array_size > GM_MAX_IEP)
+    if (d->iep_vertex_num < 0 || d->iep_vertex_num > GM_MAX_IEP_VERTEX || d->iep_array_size < 0 || d->iep_array_size > GM_MAX_IEP)
         return fail(GM_ERR_INVALID, "gm_plan_create: IEP tables too large");
     if (k > 0 && (d->iep_vertex_num == 0 || d->iep_array_size == 0)) return fail(GM_ERR_INVALID, "gm_plan_create: IEP depth without tables");
     if (d->mode != GM_MODE_PATTERN && d->mode != GM_MODE_CLIQUE) return fail(GM_ERR_INVALID, "gm_plan_create: unknown mode");
@@ -650,6 +688,11 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     Counters* ctr_bm_total = dc.ctr + 3;
     Counters* ctr_blk = dc.ctr + 4;
     Counters* ctr_blk_total = dc.ctr + 5;
+    // Donation is a pattern-rule mechanism; the clique launches run without
+    // the queue (the degree orientation and the bitmap engines bound a
+    // clique root's work).
+    a.queue = nullptr;
+    a.queue_cap = 0;
     for (int c = 0; c < np_clq; ++c) {
         const int evl = 2 + 6 * c, evb = evl + 2, evk = evl + 4;
         const gm_plan* p = plans[order[np_pat + c]];
@@ -660,15 +703,13 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         a.ctr = ctr2;
         a.root_min_outdeg = 0;
         if (use_bitmap) {
-            HIP_TRY(grow(&dc.bm_rows, &dc.bm_bytes,
-                         (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long)));
             BmArgs b;
             b.row = g->d_row;
             b.col = g->d_col;
             b.nv = g->n;
             b.k = p->host.n;
             b.max_d = bm_max;
-            b.rows_scratch = (unsigned long long*)dc.bm_rows;
+            b.rows_scratch = nullptr;
             b.nchunks = ((int64_t)g->n + kBmChunk - 1) / kBmChunk;
             b.out = cell;
             b.ctr = ctr_bm;
@@ -688,10 +729,28 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
                     big_dyn = (size_t)need * sizeof(unsigned long long);
                 }
             }
+            // Roots of out-degree up to 64 (one word per row) go to the
+            // small-root kernel; the per-wave kernel takes the roots of
+            // out-degree above 64 that the block kernel does not, and is
+            // launched only when such roots can exist.
+            b.small_max_d = std::min(kSmallD, bm_max);
+            b.tri = g->tri;
+            const int32_t dmax_bm = std::min<int32_t>(g->max_deg, bm_max);
+            bool need_wave = false;
+            if (dmax_bm > b.small_max_d) {
+                if (b.big_cap_words <= 0 || b.big_min_d > b.small_max_d) need_wave = true;
+                else if ((int64_t)dmax_bm * ((dmax_bm + 63) / 64) > b.big_cap_words) need_wave = true;
+            }
             hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_bm,
                                (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
             HIP_TRY(hipEventRecord(dc.evpool[evb], s));
-            HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
+            HIP_TRY(launch_bitmap_small(p->host.n, cus, b, s));
+            if (need_wave) {
+                HIP_TRY(grow(&dc.bm_rows, &dc.bm_bytes,
+                             (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long)));
+                b.rows_scratch = (unsigned long long*)dc.bm_rows;
+                HIP_TRY(launch_bitmap(p->host.n, bm_blocks(cus), b, s));
+            }
             if (b.big_cap_words > 0) {
                 // The large roots: their own counters and events, so their share is reported.
                 hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_blk,
@@ -699,7 +758,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
                 BmArgs bb = b;
                 bb.ctr = ctr_blk;
                 HIP_TRY(hipEventRecord(dc.evpool[evk], s));
-                HIP_TRY(launch_bitmap_big(p->host.n, big_blocks(p->host.n, cus, big_dyn), big_dyn, bb, s));
+                HIP_TRY(launch_bitmap_big(p->host.n, b.tri, big_blocks(p->host.n, b.tri, cus, big_dyn), big_dyn, bb, s));
                 HIP_TRY(hipEventRecord(dc.evpool[evk + 1], s));
                 ev_block.push_back(evk);
                 hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm, ctr_blk);
@@ -765,7 +824,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     if (c.timed_out) {
         // Slices left in the queue keep their ready flags: clear them so the
         // next launch starts from an all-clear queue.
-        if (a.queue) HIP_TRY(hipMemset(a.queue, 0, sizeof(SliceItem) * kQueueCap));
+        if (np_pat > 0 && dc.queue) HIP_TRY(hipMemset(dc.queue, 0, sizeof(SliceItem) * kQueueCap));
         return fail(GM_ERR_TIMEOUT, "gm_count: the time limit passed before the count finished");
     }
     // Scatter back to raw_sums[graph * np + plan] in the caller's plan order.
@@ -901,6 +960,12 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         a.slice_len = slice_len_of(t);
         a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
     }
+    if (t && t->time_limit_ms > 0) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess)
+            return bail(GM_ERR_HIP, "gm_batch_create: device query failed");
+        a.time_limit_ticks = (int64_t)t->time_limit_ms * (int64_t)std::max(khz, 1);
+    }
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
     a.root_begin = (hgedge[0] + root_stride - 1) / root_stride * root_stride;
     a.root_end = hgedge[ng];
@@ -948,6 +1013,12 @@ int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats)
     HIP_TRY(hipMemcpyAsync(&c, b->d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (c.overflow) return fail(GM_ERR_INVALID, "gm_batch_run: an intersection exceeded its scratch slot");
+    if (c.timed_out) {
+        // The batch's own queue keeps the ready flags of slices nobody took:
+        // clear them so the next replay starts from an all-clear queue.
+        if (b->d_queue) HIP_TRY(hipMemset(b->d_queue, 0, sizeof(SliceItem) * kQueueCap));
+        return fail(GM_ERR_TIMEOUT, "gm_batch_run: the time limit passed before the count finished");
+    }
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     if (stats) {

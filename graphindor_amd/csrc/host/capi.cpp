@@ -120,6 +120,19 @@ GS_API void* gs_graph_orient_degree(void* gp) {
     return out;
 }
 
+GS_API void* gs_graph_orient_degree_rank(void* gp) {
+    Graph* out = nullptr;
+    degree_rank_orientation((Graph*)gp, out);
+    fflush(stdout);
+    return out;
+}
+
+GS_API void* gs_graph_relabel_degree(void* gp, int ascending) {
+    Graph* out = nullptr;
+    degree_relabel((Graph*)gp, out, ascending != 0);
+    return out;
+}
+
 GS_API void gs_set_max_intersection_size(int32_t m) { VertexSet::max_intersection_size = m; }
 
 // ---- schedules -------------------------------------------------------------

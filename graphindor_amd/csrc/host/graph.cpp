@@ -209,6 +209,94 @@ void degree_orientation_init(Graph* original_g, Graph*& g) {
     printf("Maximum degree after orientation: %ld\n", (long)max_out);
 }
 
+// degree_orientation_init with the vertices renumbered by their (degree, id)
+// rank, the order the text loader's oriented_type 2 sorts by
+// (src/dataloader.cpp:279-293): vertex r of the result is the r-th vertex in
+// that order, and each edge is kept once, from its lower rank to its higher.
+// So every row holds only larger ids (N(r) is a subset of (r, n)), the order
+// in which a row lists its neighbours is also their rank order, and the local
+// graph of any row is upper-triangular in list positions -- what the bitmap
+// clique engine exploits.  Counts are those of the original graph: the
+// renumbering is an isomorphism.
+void degree_rank_orientation(Graph* original_g, Graph*& g) {
+    const v_index_t n = original_g->v_cnt;
+    std::vector<e_index_t> deg(n);
+    e_index_t max_deg = 0;
+    for (v_index_t u = 0; u < n; ++u) {
+        deg[u] = original_g->vertex[u + 1] - original_g->vertex[u];
+        max_deg = std::max(max_deg, deg[u]);
+    }
+    // Counting sort by degree, stable in id: by_rank[r] = old id of rank r.
+    std::vector<e_index_t> start((size_t)max_deg + 2, 0);
+    for (v_index_t u = 0; u < n; ++u) ++start[(size_t)deg[u] + 1];
+    for (size_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+    std::vector<v_index_t> by_rank(n), rank(n);
+    for (v_index_t u = 0; u < n; ++u) {
+        const v_index_t r = (v_index_t)start[(size_t)deg[u]]++;
+        by_rank[r] = u;
+        rank[u] = r;
+    }
+    g = new Graph();
+    g->v_cnt = n;
+    g->tri_cnt = original_g->tri_cnt;
+    g->vertex = new e_index_t[(size_t)n + 1];
+    e_index_t total = 0;
+    for (v_index_t r = 0; r < n; ++r) {
+        g->vertex[r] = total;
+        const v_index_t u = by_rank[r];
+        for (e_index_t i = original_g->vertex[u]; i < original_g->vertex[u + 1]; ++i)
+            if (rank[original_g->edge[i]] > r) ++total;
+    }
+    g->vertex[n] = total;
+    g->e_cnt = total;
+    g->edge = new v_index_t[total > 0 ? (size_t)total : 1];
+    e_index_t max_out = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : max_out)
+    for (v_index_t r = 0; r < n; ++r) {
+        const v_index_t u = by_rank[r];
+        e_index_t p = g->vertex[r];
+        for (e_index_t i = original_g->vertex[u]; i < original_g->vertex[u + 1]; ++i) {
+            const v_index_t w = rank[original_g->edge[i]];
+            if (w > r) g->edge[p++] = w;
+        }
+        std::sort(g->edge + g->vertex[r], g->edge + p);
+        max_out = std::max(max_out, p - g->vertex[r]);
+    }
+    printf("Maximum degree after orientation: %ld\n", (long)max_out);
+}
+
+// The vertices renumbered in (degree, id) order -- ascending for
+// oriented_type 2, descending for oriented_type 1 of the text loader
+// (src/dataloader.cpp:279-293) -- with every edge kept in both directions.
+// Pattern counts are those of the original graph (an isomorphism); what
+// changes is which endpoint the schedule's restrictions (emb[j] < emb[i])
+// let through first, and so the work of a count.
+void degree_relabel(Graph* original_g, Graph*& g, bool ascending) {
+    const v_index_t n = original_g->v_cnt;
+    std::vector<v_index_t> order(n), rank(n);
+    for (v_index_t u = 0; u < n; ++u) order[u] = u;
+    auto deg = [&](v_index_t u) { return original_g->vertex[u + 1] - original_g->vertex[u]; };
+    std::stable_sort(order.begin(), order.end(), [&](v_index_t a, v_index_t b) {
+        return ascending ? deg(a) < deg(b) : deg(a) > deg(b);
+    });
+    for (v_index_t r = 0; r < n; ++r) rank[order[r]] = r;
+    g = new Graph();
+    g->v_cnt = n;
+    g->e_cnt = original_g->e_cnt;
+    g->tri_cnt = original_g->tri_cnt;
+    g->vertex = new e_index_t[(size_t)n + 1];
+    g->edge = new v_index_t[g->e_cnt > 0 ? (size_t)g->e_cnt : 1];
+    g->vertex[0] = 0;
+    for (v_index_t r = 0; r < n; ++r) g->vertex[r + 1] = g->vertex[r] + deg(order[r]);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (v_index_t r = 0; r < n; ++r) {
+        const v_index_t u = order[r];
+        e_index_t p = g->vertex[r];
+        for (e_index_t i = original_g->vertex[u]; i < original_g->vertex[u + 1]; ++i) g->edge[p++] = rank[original_g->edge[i]];
+        std::sort(g->edge + g->vertex[r], g->edge + p);
+    }
+}
+
 unsigned long long gpu_pattern_matching(Graph* g, const Schedule_IEP& s, GpuRunInfo* info) {
     return run_device(g, s, GM_MODE_PATTERN, info);
 }

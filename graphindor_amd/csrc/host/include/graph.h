@@ -65,6 +65,12 @@ void reduce_edges_for_clique(Graph& g);
 // the CSR entries (src/graph.cpp:821-859).  Each k-clique is then one chain
 // of out-neighbours, and out-degrees stay near sqrt(2|E|) on skewed graphs.
 void degree_orientation_init(Graph* original_g, Graph*& g);
+// degree_orientation_init with vertices renumbered by (degree, id) rank, so
+// that every kept edge runs from a smaller id to a larger one (same counts).
+void degree_rank_orientation(Graph* original_g, Graph*& g);
+// The symmetric graph with vertices renumbered by (degree, id), ascending or
+// descending (the text loader's oriented_type 2 / 1); counts are unchanged.
+void degree_relabel(Graph* original_g, Graph*& g, bool ascending);
 
 // Device drivers replacing the file-local ones of gpu/gpu_graph.cu:22-51,
 // gpu/gpu_mc.cu:29-59 and gpu/gpu_clique.cu:366-570.  They return the count

@@ -151,6 +151,8 @@ def gs():
             ("gs_graph_count_triangles", [VP], ctypes.c_int64),
             ("gs_graph_reduce_for_clique", [VP], None),
             ("gs_graph_orient_degree", [VP], VP),
+            ("gs_graph_orient_degree_rank", [VP], VP),
+            ("gs_graph_relabel_degree", [VP, ctypes.c_int], VP),
             ("gs_set_max_intersection_size", [ctypes.c_int32], None),
             ("gs_set_restrict_validation", [ctypes.c_int], None),
             ("gs_schedule_new", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -288,6 +290,16 @@ class Graph:
     def orient_by_degree(self):
         """New graph: each edge once, from the smaller (degree, id) end."""
         return Graph(gs().gs_graph_orient_degree(self.h))
+
+    def relabel_by_degree(self, ascending=True):
+        """New graph: the same edges, vertices renumbered by (degree, id)
+        ascending (or descending); every pattern count is unchanged."""
+        return Graph(gs().gs_graph_relabel_degree(self.h, 1 if ascending else 0))
+
+    def orient_by_degree_rank(self):
+        """New graph: vertices renumbered by (degree, id) rank, each edge once
+        from the lower rank to the higher (every row holds larger ids only)."""
+        return Graph(gs().gs_graph_orient_degree_rank(self.h))
 
     def set_tri_cnt(self, t):
         gs().gs_graph_set_tri_cnt(self.h, t)
@@ -502,8 +514,15 @@ def iep_head_multiplicity(n, adj, pairs, k):
 
 
 def motifs(k):
-    buf = ctypes.create_string_buffer(k * k * 256)
+    """Connected k-vertex patterns in MotifGenerator order (n*n adjacency strings)."""
+    cap = k * k * 256
+    buf = ctypes.create_string_buffer(cap)
     n = gs().gs_motifs(k, buf, len(buf))
+    if n < 0:  # the buffer was too small: -n bytes are needed
+        buf = ctypes.create_string_buffer(-n)
+        n = gs().gs_motifs(k, buf, len(buf))
+    if n < 0:
+        raise RuntimeError(f"gs_motifs({k}) failed ({n})")
     raw = buf.raw[: n * k * k].decode()
     return [raw[i * k * k:(i + 1) * k * k] for i in range(n)]
 

@@ -45,8 +45,13 @@ extern "C" {
 /* Prefix-table length for GM_MAX_PATTERN: n(n-1)/2 + 1
  * (src/schedule_IEP.cpp:38-39). */
 #define GM_MAX_PREFIX (GM_MAX_PATTERN * (GM_MAX_PATTERN - 1) / 2 + 1)
-/* Upper bound on IEP table entries: Bell(5) groups x 5 blocks. */
-#define GM_MAX_IEP 64
+/* IEP table bounds for an IEP tail of k <= GM_MAX_PATTERN - 2 = 5 vertices
+ * (src/schedule_IEP.cpp:1762-1789 allows k up to n-2).  The vertex table
+ * has one row per non-empty subset of the tail: at most 2^5 - 1 = 31.  The
+ * answer array has one entry per block of every set partition of the tail:
+ * sum of block counts over the partitions of 5 = Bell(6) - Bell(5) = 151. */
+#define GM_MAX_IEP_VERTEX 32
+#define GM_MAX_IEP 152
 
 /* Counting semantics of a plan. */
 #define GM_MODE_PATTERN 0 /* gpu_graph / gpu_mc: restrictions, IEP, embedding subtraction
@@ -78,11 +83,11 @@ typedef struct gm_plan_desc {
     const int32_t* restrict_next;      /* [P] */
     const int32_t* restrict_index;     /* [P] emb[i] must be < emb[restrict_index] */
     const uint8_t* only_need_size;     /* [P] size-only intersection */
-    int32_t iep_vertex_num;            /* I */
+    int32_t iep_vertex_num;            /* I <= GM_MAX_IEP_VERTEX */
     const int32_t* iep_vertex_id;      /* [I] prefix whose size feeds the IEP */
     const uint8_t* iep_vertex_flag;    /* [I] 1: ans = |S| - coef; 0: ans = |S \ emb| */
     const int32_t* iep_vertex_coef;    /* [I] */
-    int32_t iep_array_size;            /* A */
+    int32_t iep_array_size;            /* A <= GM_MAX_IEP */
     const int32_t* iep_coef;           /* [A] group coefficient on the closing entry */
     const uint8_t* iep_flag;           /* [A] 1 closes a product group */
     const int32_t* iep_ans_pos;        /* [A] index into ans[] */

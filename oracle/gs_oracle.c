@@ -64,7 +64,7 @@ typedef struct {
     or_set* sets;     /* P + n (slot P+depth: vertex-induced scratch) */
     uint32_t emb[GM_MAX_PATTERN + 1];
     int emb_n;
-    int64_t ans[GM_MAX_IEP];
+    int64_t ans[GM_MAX_IEP_VERTEX];
     uint64_t sum;
     uint64_t bytes;
     uint8_t size_only[GM_MAX_PREFIX];
@@ -158,6 +158,44 @@ static int tail_depth_of(const gm_plan_desc* p) {
     return D;
 }
 
+/* Sorted intersection of a[0:na) and b[0:nb) into out (NULL: count only).
+ * Lists of comparable length are merged; when one is at least 32 times
+ * longer, each element of the shorter is found in it by galloping from the
+ * previous hit (hub neighbour lists), which gives the same elements. */
+static int32_t isect(const int32_t* a, int32_t na, const int32_t* b, int32_t nb, int32_t* out) {
+    int32_t k = 0;
+    if ((int64_t)na * 32 <= nb || (int64_t)nb * 32 <= na) {
+        const int32_t* s = na <= nb ? a : b;
+        const int32_t* l = na <= nb ? b : a;
+        const int32_t ns = na <= nb ? na : nb, nl = na <= nb ? nb : na;
+        int32_t pos = 0;
+        for (int32_t i = 0; i < ns && pos < nl; ++i) {
+            const int32_t x = s[i];
+            int32_t step = 1, hi = pos;
+            while (hi < nl && l[hi] < x) { pos = hi + 1; hi += step; step <<= 1; }
+            if (hi > nl) hi = nl;
+            while (pos < hi) {  /* first index in [pos, hi) with l[.] >= x */
+                int32_t mid = pos + ((hi - pos) >> 1);
+                if (l[mid] < x) pos = mid + 1;
+                else hi = mid;
+            }
+            if (pos < nl && l[pos] == x) {
+                if (out) out[k] = x;
+                ++k;
+                ++pos;
+            }
+        }
+        return k;
+    }
+    int32_t i = 0, j = 0;
+    while (i < na && j < nb) {
+        if (a[i] < b[j]) ++i;
+        else if (a[i] > b[j]) ++j;
+        else { if (out) out[k] = a[i]; ++k; ++i; ++j; }
+    }
+    return k;
+}
+
 static void build(or_ctx* c, int pid, const int32_t* in, int32_t n_in) {
     int f = c->p->father_prefix_id[pid];
     if (f == -1) {
@@ -167,24 +205,16 @@ static void build(or_ctx* c, int pid, const int32_t* in, int32_t n_in) {
     }
     const int32_t* a = c->sets[f].ptr;
     int32_t na = c->sets[f].len;
-    int32_t i = 0, j = 0, k = 0;
+    int32_t k;
     if (c->gpu_rule && c->size_only[pid]) {
-        while (i < na && j < n_in) {
-            if (a[i] < in[j]) ++i;
-            else if (a[i] > in[j]) ++j;
-            else { ++k; ++i; ++j; }
-        }
+        k = isect(a, na, in, n_in, NULL);
         c->sets[pid].ptr = NULL;
         c->sets[pid].len = k;
         c->bytes += 4ull * (uint64_t)(na + n_in);
         return;
     }
     int32_t* out = c->store + (int64_t)pid * c->cap;
-    while (i < na && j < n_in) {
-        if (a[i] < in[j]) ++i;
-        else if (a[i] > in[j]) ++j;
-        else { out[k++] = a[i]; ++i; ++j; }
-    }
+    k = isect(a, na, in, n_in, out);
     c->sets[pid].ptr = out;
     c->sets[pid].len = k;
     c->bytes += 4ull * (uint64_t)(na + n_in);
@@ -309,7 +339,7 @@ static void ctx_free(or_ctx* c) {
 
 static int plan_ok(const gm_plan_desc* p) {
     return p && p->size >= 2 && p->size <= GM_MAX_PATTERN && p->total_prefix_num <= GM_MAX_PREFIX &&
-           p->iep_vertex_num <= GM_MAX_IEP && p->iep_array_size <= GM_MAX_IEP;
+           p->iep_vertex_num <= GM_MAX_IEP_VERTEX && p->iep_array_size <= GM_MAX_IEP;
 }
 
 /* Raw sum (before the redundancy division) with the CPU engine's rule,
@@ -356,46 +386,56 @@ static int32_t edge_source(const or_graph* g, int64_t e) {
     return lo;
 }
 
-/* Raw sum with the device rule; *alg_bytes receives the byte model. */
-uint64_t oracle_count_edge_rooted(const or_graph* g, const gm_plan_desc* p, int threads, uint64_t* alg_bytes) {
-    if (!plan_ok(p)) return 0;
+/* One root edge e = (v0, edge[e]) under the device rule: its sum and bytes go
+ * to c->sum and c->bytes. */
+static void edge_root(or_ctx* c, const or_graph* g, const gm_plan_desc* p, int32_t v0, int64_t e) {
+    uint32_t v1 = (uint32_t)g->edge[e];
+    if (p->restrict_last[1] != -1 && (uint32_t)v0 <= v1) return;
+    c->bytes += 24ull;
+    c->emb_n = 0; /* root-level builds (depths 0 and 1) */
+    const int32_t* n0 = g->edge + g->vertex[v0];
+    int32_t d0 = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
+    for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, n0, d0);
+    const int32_t* n1 = g->edge + g->vertex[v1];
+    int32_t d1 = (int32_t)(g->vertex[v1 + 1] - g->vertex[v1]);
+    for (int pid = p->last[1]; pid != -1; pid = p->next[pid]) {
+        build(c, pid, n1, d1);
+        if (c->sets[pid].len == 0 && pid < p->basic_prefix_num) return;
+    }
+    c->emb[0] = (uint32_t)v0;
+    c->emb[1] = v1;
+    c->emb_n = 2;
+    if (p->mode == GM_MODE_CLIQUE) {
+        extern void oracle_clique_dfs(or_ctx*, int);
+        oracle_clique_dfs(c, 2);
+    } else {
+        dfs(c, 2);
+    }
+}
+
+/* Raw sum with the device rule over the root edges e with e % root_stride
+ * == 0 (the device's sampling rule, gm_tuning.root_stride; 1 counts every
+ * edge); *alg_bytes receives the byte model of those roots. */
+uint64_t oracle_count_edge_rooted_strided(const or_graph* g, const gm_plan_desc* p, int threads, int64_t root_stride,
+                                          uint64_t* alg_bytes) {
+    if (!plan_ok(p) || root_stride < 1) return 0;
     uint64_t total = 0, bytes = 0;
-    const int64_t chunk = 256;
-    int64_t nchunks = (g->nnz + chunk - 1) / chunk;
+    const int64_t nroots = (g->nnz + root_stride - 1) / root_stride;
+    /* sampled roots per task: 256, fewer for sparse samples */
+    const int64_t chunk = nroots >= 256 * 1024 ? 256 : (nroots >= 1024 ? nroots / 1024 : 1);
+    int64_t nchunks = (nroots + chunk - 1) / chunk;
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel reduction(+ : total, bytes)
     {
         or_ctx* c = ctx_new(g, p, 1);
 #pragma omp for schedule(dynamic)
         for (int64_t ch = 0; ch < nchunks; ++ch) {
-            int64_t e0 = ch * chunk, e1 = e0 + chunk < g->nnz ? e0 + chunk : g->nnz;
-            int32_t v0 = edge_source(g, e0);
-            for (int64_t e = e0; e < e1; ++e) {
+            int64_t r0 = ch * chunk, r1 = r0 + chunk < nroots ? r0 + chunk : nroots;
+            int32_t v0 = edge_source(g, r0 * root_stride);
+            for (int64_t ri = r0; ri < r1; ++ri) {
+                const int64_t e = ri * root_stride;
                 while (g->vertex[v0 + 1] <= e) ++v0;
-                uint32_t v1 = (uint32_t)g->edge[e];
-                if (p->restrict_last[1] != -1 && (uint32_t)v0 <= v1) continue;
-                c->bytes += 24ull;
-                c->emb_n = 0; /* root-level builds (depths 0 and 1) */
-                const int32_t* n0 = g->edge + g->vertex[v0];
-                int32_t d0 = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
-                for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, n0, d0);
-                const int32_t* n1 = g->edge + g->vertex[v1];
-                int32_t d1 = (int32_t)(g->vertex[v1 + 1] - g->vertex[v1]);
-                int zero = 0;
-                for (int pid = p->last[1]; pid != -1; pid = p->next[pid]) {
-                    build(c, pid, n1, d1);
-                    if (c->sets[pid].len == 0 && pid < p->basic_prefix_num) { zero = 1; break; }
-                }
-                if (zero) continue;
-                c->emb[0] = (uint32_t)v0;
-                c->emb[1] = v1;
-                c->emb_n = 2;
-                if (p->mode == GM_MODE_CLIQUE) {
-                    extern void oracle_clique_dfs(or_ctx*, int);
-                    oracle_clique_dfs(c, 2);
-                } else {
-                    dfs(c, 2);
-                }
+                edge_root(c, g, p, v0, e);
             }
         }
         total += c->sum;
@@ -404,6 +444,41 @@ uint64_t oracle_count_edge_rooted(const or_graph* g, const gm_plan_desc* p, int 
     }
     if (alg_bytes) *alg_bytes = bytes;
     return total;
+}
+
+/* A batch of ng graphs held as one CSR with global vertex ids, graph i owning
+ * the root edges [gedge[i], gedge[i+1]) (the layout gm_count_batch takes):
+ * sums[i] receives graph i's raw sum over its root edges e with
+ * e % root_stride == 0 -- absolute edge indices, as on the device, so the
+ * sample does not depend on how the batch is cut. */
+void oracle_count_batch_strided(const or_graph* g, const int64_t* gedge, int32_t ng, const gm_plan_desc* p,
+                                int threads, int64_t root_stride, uint64_t* sums) {
+    for (int32_t i = 0; i < ng; ++i) sums[i] = 0;
+    if (!plan_ok(p) || root_stride < 1 || ng < 1) return;
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel
+    {
+        or_ctx* c = ctx_new(g, p, 1);
+#pragma omp for schedule(dynamic, 1)
+        for (int32_t gi = 0; gi < ng; ++gi) {
+            const int64_t lo = (gedge[gi] + root_stride - 1) / root_stride * root_stride;
+            c->sum = 0;
+            if (lo < gedge[gi + 1]) {
+                int32_t v0 = edge_source(g, lo);
+                for (int64_t e = lo; e < gedge[gi + 1]; e += root_stride) {
+                    while (g->vertex[v0 + 1] <= e) ++v0;
+                    edge_root(c, g, p, v0, e);
+                }
+            }
+            sums[gi] = c->sum;
+        }
+        ctx_free(c);
+    }
+}
+
+/* Raw sum with the device rule; *alg_bytes receives the byte model. */
+uint64_t oracle_count_edge_rooted(const or_graph* g, const gm_plan_desc* p, int threads, uint64_t* alg_bytes) {
+    return oracle_count_edge_rooted_strided(g, p, threads, 1, alg_bytes);
 }
 
 /* gpu_clique's DFS: no restrictions, no membership test, the last level

@@ -38,6 +38,12 @@ def lib():
             getattr(L, name).argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_uint64)]
             getattr(L, name).restype = ctypes.c_uint64
+        L.oracle_count_edge_rooted_strided.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
+                                                       ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_count_edge_rooted_strided.restype = ctypes.c_uint64
+        L.oracle_count_batch_strided.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+        L.oracle_count_batch_strided.restype = None
         L.oracle_count_clique_hybrid.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_void_p, ctypes.c_int,
                                                   ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
         L.oracle_count_clique_hybrid.restype = ctypes.c_uint64
@@ -46,6 +52,12 @@ def lib():
         L.oracle_brute_force.restype = ctypes.c_uint64
         _lib = L
     return _lib
+
+
+def ref_available():
+    """True when the reference's CPU path was built here (checked without
+    loading it: the library is only loaded by the tests that call into it)."""
+    return os.path.exists(os.path.join(HERE, "_ref", "libgsref.so"))
 
 
 def ref():
@@ -102,6 +114,23 @@ def count_edge_rooted(csr, desc_ptr, threads=0):
     s = lib().oracle_count_edge_rooted(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p), threads,
                                        ctypes.byref(b))
     return s, b.value
+
+
+def count_edge_rooted_strided(csr, desc_ptr, root_stride, threads=0):
+    """Device rule over the root edges e with e % root_stride == 0: (raw sum, bytes)."""
+    b = ctypes.c_uint64(0)
+    s = lib().oracle_count_edge_rooted_strided(ctypes.byref(csr.g), ctypes.cast(desc_ptr, ctypes.c_void_p), threads,
+                                               root_stride, ctypes.byref(b))
+    return s, b.value
+
+
+def count_batch_strided(csr, graph_edge_offsets, desc_ptr, root_stride, threads=0):
+    """Per-graph raw sums of a batch CSR over its root edges e % root_stride == 0."""
+    off = np.ascontiguousarray(graph_edge_offsets, dtype=np.int64)
+    out = np.zeros(len(off) - 1, dtype=np.uint64)
+    lib().oracle_count_batch_strided(ctypes.byref(csr.g), off.ctypes.data, len(off) - 1,
+                                     ctypes.cast(desc_ptr, ctypes.c_void_p), threads, root_stride, out.ctypes.data)
+    return [int(x) for x in out]
 
 
 def count_clique(csr, desc_ptr, threads=0):

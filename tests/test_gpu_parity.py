@@ -242,23 +242,33 @@ def test_reference_mode_plans_match_oracle():
         N.set_restrict_validation("exact")
 
 
+def oriented(g, how):
+    """One direction per edge: by (degree, id) ("degree"), by (degree, id) with
+    the vertices renumbered in that order ("rank": every row above its own id),
+    or toward smaller ids ("id", the reference's reduce_edges_for_clique)."""
+    if how == "degree":
+        d = g.orient_by_degree()
+    elif how == "rank":
+        d = g.orient_by_degree_rank()
+    else:
+        v, e = g.csr()
+        return reduce_for_clique(v.copy(), e.copy())
+    v, e = d.csr()
+    return v.copy(), e.copy()
+
+
 def test_degree_oriented_cliques():
     """The clique rule on degree-oriented graphs (reference
-    degree_orientation_init) counts what the id-oriented reduction counts."""
+    degree_orientation_init, with and without renumbering by rank) counts
+    what the id-oriented reduction counts."""
     for scale, seed in ((12, 4), (14, 1)):
         g = N.Graph.rmat(scale, 16, seed)
-        d = g.orient_by_degree()
-        dv, de = d.csr()
-        dv, de = dv.copy(), de.copy()
-        g.reduce_for_clique()
-        rv, re_ = g.csr()
-        rv, re_ = rv.copy(), re_.copy()
+        graphs = [oriented(g, how) for how in ("degree", "rank", "id")]
         for k in (3, 4, 5, 6):
             s = N.Schedule.plain(k, clique_adj(k))
             s.redundancy = 1
-            a = gpu_vs_oracle(dv, de, s, N.GM_MODE_CLIQUE)
-            b = gpu_vs_oracle(rv, re_, s, N.GM_MODE_CLIQUE)
-            assert a == b
+            got = [gpu_vs_oracle(v, e, s, N.GM_MODE_CLIQUE) for v, e in graphs]
+            assert got[0] == got[1] == got[2]
 
 
 def test_clique_tail_with_large_loop_sets():
@@ -289,14 +299,14 @@ def test_clique_tail_with_large_loop_sets():
         gpu_vs_oracle(rv, re_, s, N.GM_MODE_CLIQUE)
 
 
+@pytest.mark.parametrize("how", ["degree", "rank", "id"])
 @pytest.mark.parametrize("dmax", [16, 64, 200, 1024])
-def test_bitmap_threshold_does_not_change_counts(dmax):
+def test_bitmap_threshold_does_not_change_counts(dmax, how):
     """Roots split between the local-bitmap and list engines at several
-    out-degree bounds (word counts 1, 4 and 16 all exercised)."""
+    out-degree bounds (word counts 1, 4 and 16 all exercised), on rows that
+    are triangular in either direction or not at all."""
     g = N.Graph.rmat(13, 24, 6)
-    d = g.orient_by_degree()
-    v, e = d.csr()
-    v, e = v.copy(), e.copy()
+    v, e = oriented(g, how)
     dg = N.DeviceGraph(v, e)
     csr = O.CSR(v, e)
     for k in (3, 4, 5, 6, 7):
@@ -312,13 +322,12 @@ def test_bitmap_threshold_does_not_change_counts(dmax):
             assert (raw, st.alg_bytes) == want, (k, block)
 
 
-def test_bitmap_wide_rows():
+@pytest.mark.parametrize("how", ["degree", "rank", "id"])
+def test_bitmap_wide_rows(how):
     """Out-degrees between 256 and 1024 after orientation: rows of up to 16
     words held in global scratch."""
     v, e = random_graph(700, 0.5, 12)
-    d = N.Graph.from_csr(v, e).orient_by_degree()
-    dv, de = d.csr()
-    dv, de = dv.copy(), de.copy()
+    dv, de = oriented(N.Graph.from_csr(v, e), how)
     assert np.diff(dv).max() > 256
     dg = N.DeviceGraph(dv, de)
     for k in (3, 4, 5):

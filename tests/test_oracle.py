@@ -20,7 +20,7 @@ import pytest
 from graphindor_amd import native as N
 from oracle import oracle as O
 from tests.common import (MOTIFS4, PATTERNS, automorphisms, clique_adj, complete_graph, copies_in_complete,
-                          random_graph, reduce_for_clique)
+                          random_graph, reduce_for_clique, root_edge_brute)
 
 
 def schedule_for(csr, n, adj, tri=0, **kw):
@@ -129,6 +129,19 @@ def test_rules_match_brute_force(seed, name):
         assert all_rules(csr, n, adj, iep=iep) == (want, want)
 
 
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "diamond"])
+def test_rules_match_brute_force_on_a_hub(name):
+    """A hub of degree 300 over leaves of degree ~10: intersections with the
+    hub's list are 30x skewed and take the oracle's galloping search."""
+    from tests.common import hub_graph
+    n, adj = PATTERNS[name]
+    v, e = hub_graph(300, 3, extra=0.03)
+    csr = O.CSR(v, e)
+    want = O.brute_force(csr, n, adj)
+    assert want > 0
+    assert all_rules(csr, n, adj) == (want, want)
+
+
 @pytest.mark.parametrize("name", ["c4", "house", "path4"])
 def test_vertex_induced_matches_brute_force(name):
     n, adj = PATTERNS[name]
@@ -155,6 +168,85 @@ def test_k4_pendant_closed_form():
     assert all_rules(csr, *PATTERNS["k4_pendant"]) == (want, want)
 
 
+def book_closed_form(v, e, pages):
+    """Books of `pages` triangles on one edge: sum over edges uv of C(c_uv, pages),
+    with c_uv the common neighbours of u and v."""
+    n = len(v) - 1
+    nb = [set(e[v[u]:v[u + 1]].tolist()) for u in range(n)]
+    return sum(math.comb(len(nb[u] & nb[w]), pages) for u in range(n) for w in nb[u] if u < w)
+
+
+def test_iep_depth_five_plan_and_count():
+    """The 7-vertex book has five mutually non-adjacent trailing vertices: IEP
+    depth 5 and Bell(6) - Bell(5) = 151 answer entries (every tail subset
+    shares one prefix, so the vertex table has one row), within the plan's
+    bounds, and both rules count the closed form."""
+    n, adj = PATTERNS["book5"]
+    v, e = random_graph(18, 0.6, 21)
+    csr = O.CSR(v, e)
+    s = schedule_for(csr, n, adj)
+    hdr = s.dump()[:10]  # n, P, basic, restricts, k, induced, redundancy, I, A, pairs
+    assert (hdr[4], hdr[8]) == (5, 151) and 1 <= hdr[7] <= 31
+    want = book_closed_form(v, e, 5)
+    assert want > 0
+    assert all_rules(csr, n, adj) == (want, want)
+    assert all_rules(csr, n, adj, iep=False) == (want, want)
+
+
+def test_degree_rank_orientation():
+    """Renumbering by (degree, id) rank keeps one direction per edge, from the
+    smaller rank to the larger: every row is sorted and above its own id, the
+    out-degrees are those of degree_orientation_init (the renumbering is an
+    isomorphism), the edges are the original ones mapped through the rank,
+    and the clique counts equal the id-reduced graph's."""
+    g = N.Graph.rmat(11, 12, 9)
+    v, e = (x.copy() for x in g.csr())
+    n = len(v) - 1
+    rg, dg = g.orient_by_degree_rank(), g.orient_by_degree()
+    rv, re_ = (x.copy() for x in rg.csr())
+    dv = dg.csr()[0].copy()
+    assert len(re_) * 2 == len(e) and len(rv) == n + 1
+    src = np.repeat(np.arange(n), np.diff(rv))
+    assert (re_ > src).all()
+    assert all((np.diff(re_[rv[u]:rv[u + 1]]) > 0).all() for u in range(n))
+    deg = np.diff(v)
+    by_rank = np.lexsort((np.arange(n), deg))  # the r-th vertex by (degree, id)
+    assert (np.diff(rv) == np.diff(dv)[by_rank]).all()
+    rank = np.empty(n, dtype=np.int64)
+    rank[by_rank] = np.arange(n)
+    osrc = np.repeat(np.arange(n), deg)
+    lo, hi = np.minimum(rank[osrc], rank[e]), np.maximum(rank[osrc], rank[e])
+    assert sorted(zip(src.tolist(), re_.tolist())) == sorted(set(zip(lo.tolist(), hi.tolist())))
+    iv, ie = reduce_for_clique(v, e)
+    for k in (3, 4, 5):
+        d = N.Schedule.plain(k, clique_adj(k)).plan_desc(N.GM_MODE_CLIQUE)
+        assert O.count_clique(O.CSR(rv, re_), d)[0] == O.count_clique(O.CSR(iv, ie), d)[0]
+
+
+def test_clique_hybrid_rule_equals_clique_rule():
+    """The bitmap split (roots of out-degree <= dmax counted on local bitmaps)
+    changes neither the count nor, for dmax = 0, the byte tally of the plain
+    clique rule; every threshold gives the clique rule's count."""
+    v, e = random_graph(90, 0.25, 31)
+    ov, oe = reduce_for_clique(v, e)
+    csr = O.CSR(ov, oe)
+    dmax_all = int(np.diff(ov).max())
+    for k in (3, 4, 5, 6):
+        s = N.Schedule.plain(k, clique_adj(k))
+        d = s.plan_desc(N.GM_MODE_CLIQUE)
+        want, wbytes = O.count_clique(csr, d)
+        assert O.count_clique_hybrid(csr, d, 0) == (want, wbytes)
+        for dmax in (1, 4, 8, 16, dmax_all):
+            assert O.count_clique_hybrid(csr, d, dmax)[0] == want
+    k4 = O.count_clique(csr, N.Schedule.plain(4, clique_adj(4)).plan_desc(N.GM_MODE_CLIQUE))[0]
+    A = np.zeros((90, 90), dtype=np.int64)
+    for u in range(90):
+        A[u, e[v[u]:v[u + 1]]] = 1
+    brute = sum(1 for q in __import__("itertools").combinations(range(90), 4)
+                if all(A[a, b] for a, b in __import__("itertools").combinations(q, 2)))
+    assert k4 == brute
+
+
 def test_planted_mistake_is_caught():
     """Dropping the restriction of a plan (a plausible bug) changes the count,
     and the brute force exposes it."""
@@ -178,13 +270,55 @@ def test_planted_mistake_is_caught():
         assert O.count_vertex_rooted(csr, ctypes.pointer(bad2)) // s.redundancy != good
 
 
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "tailed_triangle"])
+def test_edge_rooted_stride_counts_exactly_the_multiples(name):
+    """The strided device rule sums exactly the root edges e with e % S == 0:
+    equal to a per-root brute force restated from the plan (IEP off), for
+    several strides; stride 1 is the full count."""
+    n, adj = PATTERNS[name]
+    v, e = random_graph(22, 0.35, 41)
+    csr = O.CSR(v, e)
+    s = schedule_for(csr, n, adj, iep=False)
+    d = s.plan_desc()
+    full = O.count_edge_rooted(csr, d)
+    assert O.count_edge_rooted_strided(csr, d, 1) == full
+    assert full[0] // s.redundancy == O.brute_force(csr, n, adj)
+    for S in (2, 5, 13, len(e) + 3):
+        got, _ = O.count_edge_rooted_strided(csr, d, S)
+        assert got == root_edge_brute(v, e, s, range(0, len(e), S)), S
+    # IEP on: the strided tallies of the phases of S = 1 and the full tally agree
+    si = schedule_for(csr, n, adj)
+    di = si.plan_desc()
+    assert O.count_edge_rooted_strided(csr, di, 1) == O.count_edge_rooted(csr, di)
+
+
+@pytest.mark.parametrize("stride", [1, 3, 64])
+def test_batch_strided_per_graph_sums(stride):
+    """Per-graph sums of a batch CSR (global ids, graph i owning the root
+    edges [eoff[i], eoff[i+1])) over the absolute sample e % stride == 0:
+    each equals the per-root brute force of that graph's sampled edges, and
+    they add up to the strided count of the whole CSR."""
+    g, voff, eoff = N.Graph.batch(6, 5, 5.0, 6.5, 4.0, 10.0)
+    v, e = (x.copy() for x in g.csr())
+    csr = O.CSR(v, e)
+    for name in ("tailed_triangle", "c4", "house"):
+        n, adj = PATTERNS[name]
+        s = schedule_for(csr, n, adj, iep=False)
+        d = s.plan_desc()
+        sums = O.count_batch_strided(csr, eoff, d, stride)
+        for i in range(len(eoff) - 1):
+            first = (eoff[i] + stride - 1) // stride * stride
+            assert sums[i] == root_edge_brute(v, e, s, range(first, eoff[i + 1], stride)), (name, i)
+        assert sum(sums) == O.count_edge_rooted_strided(csr, d, stride)[0]
+
+
 def test_automorphism_orders():
     assert [automorphisms(*PATTERNS[k]) for k in ("house", "c5", "k4_pendant", "c6", "prism")] == [2, 10, 6, 12, 12]
 
 
 # ---- against the reference's own CPU path, where it was built from source ----
 
-ref_missing = O.ref() is None
+ref_missing = not O.ref_available()
 
 
 @pytest.mark.skipif(ref_missing, reason="oracle/_ref not built (reference sources absent)")

@@ -146,10 +146,10 @@ def test_redundancy_and_complete_graph_counts(name):
 
 def test_motif_generator_order_and_counts():
     assert N.motifs(4) == MOTIFS4
-    assert [len(N.motifs(k)) for k in (3, 4, 5, 6)] == [2, 6, 21, 112]
+    assert [len(N.motifs(k)) for k in (3, 4, 5, 6, 7)] == [2, 6, 21, 112, 853]
 
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 def test_motif_generator_matches_reference():
     import ctypes
     for k in (3, 4, 5):
@@ -159,7 +159,7 @@ def test_motif_generator_matches_reference():
         assert [raw[i * k * k:(i + 1) * k * k] for i in range(m)] == N.motifs(k)
 
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
 def test_reference_mode_matches_live_reference(reference_mode):
     """Random connected 4-6 vertex patterns at several graph statistics.
     Skips the patterns on which the reference reads past its own IEP answer
