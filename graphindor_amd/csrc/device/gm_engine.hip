@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -267,6 +268,42 @@ hipError_t launch_bitmap(int k, int blocks, const BmArgs& b, hipStream_t s) {
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+// Static workspace of gm_count_static: counters, sums and the list engine's
+// prefix slots of a count that allocates nothing.  Each device holds its own
+// copy from module load to process end.  Host code never uses this name as
+// an address: static_workspace() looks up the device's own address once.
+static __device__ __align__(256) unsigned char scratch[GM_STATIC_WORKSPACE_BYTES];
+
+namespace {
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_ws_lookup_mu;
+unsigned char* g_ws_ptr[kMaxStaticDevices] = {};
+std::mutex g_ws_count_mu[kMaxStaticDevices];
+
+std::mutex& static_ws_mutex(int dev) { return g_ws_count_mu[dev]; }
+
+// The device address of `scratch` on `dev` (the current device), looked up
+// the first time a call runs there.
+int static_workspace(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxStaticDevices) return fail(GM_ERR_UNSUPPORTED, "gm_count_static: device index too large");
+    std::lock_guard<std::mutex> lock(g_ws_lookup_mu);
+    if (!g_ws_ptr[dev]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        void* ptr = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(GM_ERR_HIP, std::string("gm_count_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_ws_ptr[dev] = (unsigned char*)ptr;
+    }
+    *out = g_ws_ptr[dev];
+    return GM_OK;
 }
 
 }  // namespace
@@ -522,8 +559,15 @@ void gm_plan_destroy(gm_plan* p) {
     delete p;
 }
 
-int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng, const gm_plan* const* plans,
-                   int32_t np, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sums, gm_stats* stats) {
+// The counting call behind gm_count, gm_count_multi and gm_count_batch
+// (fixed_ws == nullptr: device buffers come from the per-device cache, grown
+// on demand) and gm_count_static (fixed_ws: the device address of the static
+// workspace, GM_STATIC_WORKSPACE_BYTES, from which every buffer is carved;
+// nothing is allocated, slice donation is off, no HIP event is used and the
+// list engine's grid is cut to the waves whose prefix slots fit).
+static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng, const gm_plan* const* plans,
+                      int32_t np, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sums, gm_stats* stats,
+                      unsigned char* fixed_ws) {
     if (!g || !plans || np <= 0 || !raw_sums || ng < 1) return fail(GM_ERR_INVALID, "gm_count: bad arguments");
     for (int i = 0; i < np; ++i)
         if (!plans[i]) return fail(GM_ERR_INVALID, "gm_count: null plan");
@@ -538,8 +582,10 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     }
     HIP_TRY(hipSetDevice(g->device));
     hipStream_t s = (hipStream_t)stream;
-    DeviceCache& dc = cache_for(g->device);
-    std::lock_guard<std::mutex> lock(dc.mu);
+    DeviceCache fixed_dc;  // fixed_ws: views of the static workspace, owned by nobody
+    DeviceCache& dc = fixed_ws ? fixed_dc : cache_for(g->device);
+    std::unique_lock<std::mutex> lock(fixed_ws ? static_ws_mutex(g->device) : dc.mu);
+    const auto t_host0 = std::chrono::steady_clock::now();
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
     // Default grids: every block the occupancy of the launched kernel allows
@@ -561,11 +607,27 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, &occ));
         blocks_clq = grid_blocks(cus, occ, req_waves);
     }
-    const int64_t nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
+    int64_t nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
     const int32_t cap = std::max(1, g->second_deg);
     const int32_t cap_vi = std::max(1, g->max_deg);
     const int task_max = (t && t->chunk_edges > 0) ? t->chunk_edges : kWave;
-    const int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
+    int bm_max = (t && t->bitmap_max_degree != 0) ? std::min(t->bitmap_max_degree, kBmMaxD) : kBmMaxD;
+    if (fixed_ws && bm_max > 0) {
+        // Without the per-wave bitmap kernel's global rows, the bitmap
+        // engine takes only the roots the small-root and block kernels
+        // cover; the list engine counts the rest.
+        int32_t cover = std::min(kSmallD, bm_max);
+        const int big_min = (t && t->bitmap_block_degree > 0) ? t->bitmap_block_degree : kBigMinD;
+        if (!(t && t->bitmap_block_degree < 0) && big_min <= kSmallD) {
+            int32_t cap_words = 0;
+            size_t dyn = 0;
+            HIP_TRY(big_rows_capacity(g->device, &cap_words, &dyn));
+            int32_t d = bm_max;
+            while (d > cover && (int64_t)d * ((d + 63) / 64) > cap_words) --d;
+            cover = std::max(cover, d);
+        }
+        bm_max = cover;
+    }
     const int stats_on = (flags & GM_COUNT_ALG_BYTES) ? 1 : 0;
     const int64_t root_stride = (t && t->root_stride > 1) ? t->root_stride : 1;
 
@@ -597,22 +659,64 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         if (e == hipSuccess) *have = need;
         return e;
     };
-    HIP_TRY(grow(&dc.scratch, &dc.scratch_bytes, (size_t)stride * (size_t)nwaves * sizeof(uint32_t)));
-    HIP_TRY(grow(&dc.plan_ptrs, &dc.plan_ptrs_bytes, sizeof(DevPlan*) * (size_t)np));
-    HIP_TRY(grow(&dc.gedge, &dc.gedge_bytes, sizeof(int64_t) * (size_t)(ng + 1)));
-    HIP_TRY(grow(&dc.sums, &dc.sums_bytes, sizeof(unsigned long long) * (size_t)ng * (size_t)np));
-    if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * kNumCounters));
-    if (!dc.ev0) {
-        HIP_TRY(hipEventCreate(&dc.ev0));
-        HIP_TRY(hipEventCreate(&dc.ev1));
+    // Is the list engine launched at all?  Pattern plans always; a clique
+    // plan unless the bitmap engine takes every root.
+    bool list_needed = np_pat > 0;
+    for (int c = 0; c < np_clq; ++c) {
+        const gm_plan* p = plans[order[np_pat + c]];
+        const bool bm = ng == 1 && !graph_edge_offsets && p->host.clique_chain && bm_max > 0;
+        if (!(bm && g->max_deg <= bm_max)) list_needed = true;
     }
-    // Event pairs: the fused pattern launch, then a list and a bitmap launch per clique plan.
-    const size_t nev = 2 * (size_t)(1 + 3 * np_clq);
-    while (dc.evpool.size() < nev) {
-        hipEvent_t ev = nullptr;
-        HIP_TRY(hipEventCreate(&ev));
-        dc.evpool.push_back(ev);
+    if (fixed_ws) {
+        // Carve the buffers from the static workspace; the prefix slots get
+        // what is left, and the list grid shrinks to the waves they hold.
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            unsigned char* p = fixed_ws + off;
+            off += (bytes + 255) & ~(size_t)255;
+            return (void*)p;
+        };
+        dc.ctr = (Counters*)carve(sizeof(Counters) * kNumCounters);
+        dc.plan_ptrs = carve(sizeof(DevPlan*) * (size_t)np);
+        dc.gedge = carve(sizeof(int64_t) * (size_t)(ng + 1));
+        dc.sums = carve(sizeof(unsigned long long) * (size_t)ng * (size_t)np);
+        if (off > (size_t)GM_STATIC_WORKSPACE_BYTES)
+            return fail(GM_ERR_UNSUPPORTED, "gm_count_static: the counters of this call exceed the static workspace");
+        dc.scratch = fixed_ws + off;
+        dc.scratch_bytes = (size_t)GM_STATIC_WORKSPACE_BYTES - off;
+        if (list_needed) {
+            const int64_t slot_bytes = stride * (int64_t)sizeof(uint32_t);
+            const int64_t fit_blocks = (int64_t)dc.scratch_bytes / (slot_bytes * kWavesPerBlock);
+            if (fit_blocks < 1)
+                return fail(GM_ERR_UNSUPPORTED, "gm_count_static: one block of list-engine waves needs " +
+                                                    std::to_string(slot_bytes * kWavesPerBlock) +
+                                                    " bytes of prefix slots; the static workspace has " +
+                                                    std::to_string(dc.scratch_bytes));
+            if (blocks_pat > fit_blocks) blocks_pat = (int)fit_blocks;
+            if (blocks_clq > fit_blocks) blocks_clq = (int)fit_blocks;
+            nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
+        }
+    } else {
+        HIP_TRY(grow(&dc.scratch, &dc.scratch_bytes, (size_t)stride * (size_t)nwaves * sizeof(uint32_t)));
+        HIP_TRY(grow(&dc.plan_ptrs, &dc.plan_ptrs_bytes, sizeof(DevPlan*) * (size_t)np));
+        HIP_TRY(grow(&dc.gedge, &dc.gedge_bytes, sizeof(int64_t) * (size_t)(ng + 1)));
+        HIP_TRY(grow(&dc.sums, &dc.sums_bytes, sizeof(unsigned long long) * (size_t)ng * (size_t)np));
+        if (!dc.ctr) HIP_TRY(hipMalloc(&dc.ctr, sizeof(Counters) * kNumCounters));
+        if (!dc.ev0) {
+            HIP_TRY(hipEventCreate(&dc.ev0));
+            HIP_TRY(hipEventCreate(&dc.ev1));
+        }
+        // Event pairs: the fused pattern launch, then a list and a bitmap launch per clique plan.
+        const size_t nev = 2 * (size_t)(1 + 3 * np_clq);
+        while (dc.evpool.size() < nev) {
+            hipEvent_t ev = nullptr;
+            HIP_TRY(hipEventCreate(&ev));
+            dc.evpool.push_back(ev);
+        }
     }
+    // HIP events around the launches (the static-workspace call times on the host).
+    const bool timed = fixed_ws == nullptr;
+    auto rec = [&](hipEvent_t ev) -> hipError_t { return timed ? hipEventRecord(ev, s) : hipSuccess; };
     std::vector<int> ev_list, ev_bitmap, ev_block;  // indices of start events used
     std::vector<const DevPlan*> hptrs(np);
     for (int k = 0; k < np; ++k) hptrs[k] = plans[order[k]]->d_plan;
@@ -626,7 +730,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     const int64_t ncells = (int64_t)ng * np;
     const int64_t nctr_words = (int64_t)(sizeof(Counters) * kNumCounters / sizeof(unsigned long long));
 
-    HIP_TRY(hipEventRecord(dc.ev0, s));
+    HIP_TRY(rec(dc.ev0));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, s, d_sums, ncells);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)dc.ctr, nctr_words);
     HIP_TRY(hipGetLastError());
@@ -649,7 +753,7 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     a.task_max = task_max;
     // Slice donation applies to the pattern rule; clique plans rely on the
     // degree orientation and the bitmap engine to bound a root's work.
-    if (np_pat > 0) {
+    if (np_pat > 0 && !fixed_ws) {
         const int32_t sl = slice_len_of(t);
         if (sl > 0) {
             HIP_TRY(alloc_queue(&dc.queue));
@@ -675,10 +779,10 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         a.np = np_pat;
         a.sums = d_sums;
         a.root_min_outdeg = 0;
-        HIP_TRY(hipEventRecord(dc.evpool[0], s));
+        HIP_TRY(rec(dc.evpool.empty() ? nullptr : dc.evpool[0]));
         enum_launch<GM_MODE_PATTERN>(nmax_pat, blocks_pat, s, a);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(dc.evpool[1], s));
+        HIP_TRY(rec(dc.evpool.empty() ? nullptr : dc.evpool[1]));
         ev_list.push_back(0);
     }
     // Clique plans: one at a time; on a single graph the local-bitmap engine
@@ -743,9 +847,10 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
             }
             hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)ctr_bm,
                                (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
-            HIP_TRY(hipEventRecord(dc.evpool[evb], s));
+            HIP_TRY(rec(timed ? dc.evpool[evb] : nullptr));
             HIP_TRY(launch_bitmap_small(p->host.n, cus, b, s));
             if (need_wave) {
+                if (fixed_ws) return fail(GM_ERR_HIP, "gm_count_static: internal: per-wave bitmap rows needed");
                 HIP_TRY(grow(&dc.bm_rows, &dc.bm_bytes,
                              (size_t)bm_blocks(cus) * kBmMaxD * kBmMaxW * sizeof(unsigned long long)));
                 b.rows_scratch = (unsigned long long*)dc.bm_rows;
@@ -757,14 +862,14 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
                                    (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
                 BmArgs bb = b;
                 bb.ctr = ctr_blk;
-                HIP_TRY(hipEventRecord(dc.evpool[evk], s));
+                HIP_TRY(rec(timed ? dc.evpool[evk] : nullptr));
                 HIP_TRY(launch_bitmap_big(p->host.n, b.tri, big_blocks(p->host.n, b.tri, cus, big_dyn), big_dyn, bb, s));
-                HIP_TRY(hipEventRecord(dc.evpool[evk + 1], s));
+                HIP_TRY(rec(timed ? dc.evpool[evk + 1] : nullptr));
                 ev_block.push_back(evk);
                 hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm, ctr_blk);
                 hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_blk_total, ctr_blk);
             }
-            HIP_TRY(hipEventRecord(dc.evpool[evb + 1], s));
+            HIP_TRY(rec(timed ? dc.evpool[evb + 1] : nullptr));
             ev_bitmap.push_back(evb);
             hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr_bm);
             hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, ctr_bm_total, ctr_bm);
@@ -784,16 +889,16 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         // of the list launch fails the root_min_outdeg filter -- no sum, no
         // bytes, no task -- so the launch, a 9.5 ms scan there, is skipped.
         if (!(use_bitmap && g->max_deg <= bm_max)) {
-            HIP_TRY(hipEventRecord(dc.evpool[evl], s));
+            HIP_TRY(rec(timed ? dc.evpool[evl] : nullptr));
             enum_launch<GM_MODE_CLIQUE>(nmax_clq, blocks_clq, s, a);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipEventRecord(dc.evpool[evl + 1], s));
+            HIP_TRY(rec(timed ? dc.evpool[evl + 1] : nullptr));
             ev_list.push_back(evl);
         }
         hipLaunchKernelGGL(accumulate_counters_kernel, dim3(1), dim3(1), 0, s, dc.ctr, ctr2);
         a.ctr = dc.ctr;
     }
-    HIP_TRY(hipEventRecord(dc.ev1, s));
+    HIP_TRY(rec(dc.ev1));
     std::vector<unsigned long long> hs(ncells);
     Counters c, cb, ck;
     HIP_TRY(hipMemcpyAsync(hs.data(), d_sums, sizeof(unsigned long long) * ncells, hipMemcpyDeviceToHost, s));
@@ -802,23 +907,25 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
     HIP_TRY(hipMemcpyAsync(&ck, ctr_blk_total, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
-    double list_ms = 0, bitmap_ms = 0;
-    for (int i : ev_list) {
-        float x = 0;
-        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
-        list_ms += x;
-    }
-    for (int i : ev_bitmap) {
-        float x = 0;
-        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
-        bitmap_ms += x;
-    }
-    double block_ms = 0;
-    for (int i : ev_block) {
-        float x = 0;
-        HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
-        block_ms += x;
+    if (timed) HIP_TRY(hipEventElapsedTime(&ms, dc.ev0, dc.ev1));
+    else ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
+    double list_ms = 0, bitmap_ms = 0, block_ms = 0;
+    if (timed) {
+        for (int i : ev_list) {
+            float x = 0;
+            HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+            list_ms += x;
+        }
+        for (int i : ev_bitmap) {
+            float x = 0;
+            HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+            bitmap_ms += x;
+        }
+        for (int i : ev_block) {
+            float x = 0;
+            HIP_TRY(hipEventElapsedTime(&x, dc.evpool[i], dc.evpool[i + 1]));
+            block_ms += x;
+        }
     }
     if (c.overflow) return fail(GM_ERR_INVALID, "gm_count: an intersection exceeded its scratch slot (graph/plan mismatch)");
     if (c.timed_out) {
@@ -848,6 +955,20 @@ int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t
         fill_queue_stats(c, stats);
     }
     return GM_OK;
+}
+
+int gm_count_batch(const gm_graph* g, const int64_t* graph_edge_offsets, int32_t ng, const gm_plan* const* plans,
+                   int32_t np, const gm_tuning* t, int32_t flags, void* stream, uint64_t* raw_sums, gm_stats* stats) {
+    return count_impl(g, graph_edge_offsets, ng, plans, np, t, flags, stream, raw_sums, stats, nullptr);
+}
+
+int gm_count_static(const gm_graph* g, const gm_plan* p, const gm_tuning* t, int32_t flags, void* stream,
+                    uint64_t* raw_sum, gm_stats* stats) {
+    if (!g || !p || !raw_sum) return fail(GM_ERR_INVALID, "gm_count_static: bad arguments");
+    unsigned char* ws = nullptr;
+    const int rc = static_workspace(g->device, &ws);
+    if (rc != GM_OK) return rc;
+    return count_impl(g, nullptr, 1, &p, 1, t, flags, stream, raw_sum, stats, ws);
 }
 
 int gm_count_multi(const gm_graph* g, const gm_plan* const* plans, int32_t np, const gm_tuning* t, int32_t flags,

@@ -98,6 +98,9 @@ def gm():
         lib.gm_count.argtypes = [VP, VP, ctypes.POINTER(GmTuning), ctypes.c_int32, VP,
                                  ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(GmStats)]
         lib.gm_count.restype = ctypes.c_int
+        lib.gm_count_static.argtypes = [VP, VP, ctypes.POINTER(GmTuning), ctypes.c_int32, VP,
+                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(GmStats)]
+        lib.gm_count_static.restype = ctypes.c_int
         lib.gm_count_multi.argtypes = [VP, ctypes.POINTER(VP), ctypes.c_int32, ctypes.POINTER(GmTuning), ctypes.c_int32,
                                        VP, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(GmStats)]
         lib.gm_count_multi.restype = ctypes.c_int
@@ -577,6 +580,17 @@ def count(dgraph, dplan, flags=0, **tuning):
     tun = _tuning(**tuning)
     _gm_check(gm().gm_count(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw), ctypes.byref(st)),
               "gm_count")
+    return raw.value, st
+
+
+def count_static(dgraph, dplan, flags=0, **tuning):
+    """gm_count through gm_count_static: the same raw sum, computed in the
+    library's static per-device workspace without allocating device memory."""
+    raw = ctypes.c_uint64(0)
+    st = GmStats()
+    tun = _tuning(**tuning)
+    _gm_check(gm().gm_count_static(dgraph.h, dplan.h, ctypes.byref(tun), flags, None, ctypes.byref(raw),
+                                   ctypes.byref(st)), "gm_count_static")
     return raw.value, st
 
 

@@ -167,6 +167,23 @@ void gm_plan_destroy(gm_plan* p);
 int gm_count(const gm_graph* g, const gm_plan* p, const gm_tuning* tuning, int32_t flags,
              void* stream, uint64_t* raw_sum, gm_stats* stats);
 
+/* gm_count without allocating: no reference counterpart; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Every device buffer the
+ * count needs (counters, the sum cell, the list engine's per-wave prefix
+ * slots) is carved from one static per-device workspace of
+ * GM_STATIC_WORKSPACE_BYTES; the list engine runs as many waves as have room
+ * for their prefix slots there (fewer than gm_count on graphs with long
+ * lists), slice donation is off, and stats->kernel_ms is host wall time
+ * (list_ms, bitmap_ms are 0).  Sums equal gm_count's.  Fails with
+ * GM_ERR_UNSUPPORTED when one block of waves does not fit (second-largest
+ * degree x prefixes too large).  Calls on one device share the workspace:
+ * they are serialised within a process, and calls from several processes
+ * on one GPU race. */
+#define GM_STATIC_WORKSPACE_BYTES (64u << 20)
+int gm_count_static(const gm_graph* g, const gm_plan* p, const gm_tuning* tuning, int32_t flags,
+                    void* stream, uint64_t* raw_sum, gm_stats* stats);
+
 /* Count several plans over one graph (e.g. every connected k-motif) in one
  * persistent launch; raw_sums[np].  Replaces the per-motif init + launch
  * loop of gpu/gpu_mc.cu:96-115. */
