@@ -35,6 +35,7 @@
  * loop set before the IEP depth, every IEP row reading them has flag 1), the
  * same rule the device uses.
  */
+#include <limits.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -70,6 +71,7 @@ typedef struct {
     uint8_t size_only[GM_MAX_PREFIX];
     int gpu_rule;     /* 1: edge-rooted device rule, 0: CPU engine rule */
     int32_t d1_stride, d1_offset; /* CPU rule: expand only depth-1 candidates i = offset mod stride */
+    int32_t d2_lo, d2_hi;         /* device rule: walk only the depth-2 candidates [d2_lo, d2_hi) */
     int tail_depth;   /* device rule: builds at this depth are charged as size-only */
 } or_ctx;
 
@@ -298,7 +300,12 @@ static void dfs(or_ctx* c, int depth) {
     }
     uint32_t bound = restriction_bound(c, depth, c->gpu_rule ? 0xffffffffu : (uint32_t)c->g->n);
     const int sampled = depth == 1 && c->d1_stride > 1;
-    for (int32_t i = 0; i < loop.len; ++i) {
+    int32_t i0 = 0, i1 = loop.len;
+    if (depth == 2) {
+        i0 = c->d2_lo;
+        if (c->d2_hi < i1) i1 = c->d2_hi;
+    }
+    for (int32_t i = i0; i < i1; ++i) {
         uint32_t v = (uint32_t)loop.ptr[i];
         if (bound <= v) break;
         if (sampled && i % c->d1_stride != c->d1_offset) continue;
@@ -327,6 +334,8 @@ static or_ctx* ctx_new(const or_graph* g, const gm_plan_desc* p, int gpu_rule) {
     c->sets = (or_set*)calloc((size_t)(p->total_prefix_num + p->size), sizeof(or_set));
     c->gpu_rule = gpu_rule;
     c->tail_depth = gpu_rule ? tail_depth_of(p) : -1;
+    c->d2_lo = 0;
+    c->d2_hi = INT32_MAX;
     derive_size_only(p, c->size_only);
     return c;
 }
@@ -386,6 +395,35 @@ static int32_t edge_source(const or_graph* g, int64_t e) {
     return lo;
 }
 
+/* Is depth 2 a candidate loop under the device rule (and so splittable)? */
+static int depth2_loops(const gm_plan_desc* p) {
+    const int n = p->size, k = p->in_exclusion_optimize_num;
+    if (p->mode == GM_MODE_CLIQUE) return n >= 4;
+    return n - k > 2 && !(k == 0 && n == 3);
+}
+
+/* Depth-0/1 builds of root e (bytes charged); returns the length of its
+ * depth-2 loop set, or -1 when the root is skipped or ends at depth 1. */
+static int32_t edge_root_prefixes(or_ctx* c, const or_graph* g, const gm_plan_desc* p, int32_t v0, int64_t e) {
+    uint32_t v1 = (uint32_t)g->edge[e];
+    if (p->restrict_last[1] != -1 && (uint32_t)v0 <= v1) return -1;
+    c->bytes += 24ull;
+    c->emb_n = 0;
+    const int32_t* n0 = g->edge + g->vertex[v0];
+    int32_t d0 = (int32_t)(g->vertex[v0 + 1] - g->vertex[v0]);
+    for (int pid = p->last[0]; pid != -1; pid = p->next[pid]) build(c, pid, n0, d0);
+    const int32_t* n1 = g->edge + g->vertex[v1];
+    int32_t d1 = (int32_t)(g->vertex[v1 + 1] - g->vertex[v1]);
+    for (int pid = p->last[1]; pid != -1; pid = p->next[pid]) {
+        build(c, pid, n1, d1);
+        if (c->sets[pid].len == 0 && pid < p->basic_prefix_num) return -1;
+    }
+    c->emb[0] = (uint32_t)v0;
+    c->emb[1] = v1;
+    c->emb_n = 2;
+    return c->sets[p->loop_set_prefix_id[2]].len;
+}
+
 /* One root edge e = (v0, edge[e]) under the device rule: its sum and bytes go
  * to c->sum and c->bytes. */
 static void edge_root(or_ctx* c, const or_graph* g, const gm_plan_desc* p, int32_t v0, int64_t e) {
@@ -421,27 +459,71 @@ uint64_t oracle_count_edge_rooted_strided(const or_graph* g, const gm_plan_desc*
     if (!plan_ok(p) || root_stride < 1) return 0;
     uint64_t total = 0, bytes = 0;
     const int64_t nroots = (g->nnz + root_stride - 1) / root_stride;
-    /* sampled roots per task: 256, fewer for sparse samples */
-    const int64_t chunk = nroots >= 256 * 1024 ? 256 : (nroots >= 1024 ? nroots / 1024 : 1);
-    int64_t nchunks = (nroots + chunk - 1) / chunk;
     if (threads > 0) omp_set_num_threads(threads);
+    /* A root's depth-2 candidates are split into up to 64 pieces of at least
+     * 64 candidates, so that the threads share a hub root's work; a piece
+     * rebuilds the root's prefixes, and only the first is charged for them. */
+    int32_t* pieces = (int32_t*)calloc((size_t)(nroots > 0 ? nroots : 1), sizeof(int32_t));
+    const int split = depth2_loops(p);
+    if (split) {
+#pragma omp parallel
+        {
+            or_ctx* c = ctx_new(g, p, 1);
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t ri = 0; ri < nroots; ++ri) {
+                const int64_t e = ri * root_stride;
+                int32_t L = edge_root_prefixes(c, g, p, edge_source(g, e), e);
+                int32_t k = L <= 64 ? 1 : (L + 63) / 64;
+                pieces[ri] = k > 64 ? 64 : k;
+            }
+            ctx_free(c);
+        }
+    } else {
+        for (int64_t ri = 0; ri < nroots; ++ri) pieces[ri] = 1;
+    }
+    int64_t* first = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nroots + 1));
+    first[0] = 0;
+    for (int64_t ri = 0; ri < nroots; ++ri) first[ri + 1] = first[ri] + pieces[ri];
+    const int64_t nitems = first[nroots];
 #pragma omp parallel reduction(+ : total, bytes)
     {
         or_ctx* c = ctx_new(g, p, 1);
 #pragma omp for schedule(dynamic)
-        for (int64_t ch = 0; ch < nchunks; ++ch) {
-            int64_t r0 = ch * chunk, r1 = r0 + chunk < nroots ? r0 + chunk : nroots;
-            int32_t v0 = edge_source(g, r0 * root_stride);
-            for (int64_t ri = r0; ri < r1; ++ri) {
-                const int64_t e = ri * root_stride;
-                while (g->vertex[v0 + 1] <= e) ++v0;
-                edge_root(c, g, p, v0, e);
+        for (int64_t it = 0; it < nitems; ++it) {
+            int64_t lo = 0, hi = nroots;  /* root of item it: largest ri with first[ri] <= it */
+            while (hi - lo > 1) {
+                int64_t mid = (lo + hi) / 2;
+                if (first[mid] <= it) lo = mid;
+                else hi = mid;
             }
+            const int64_t ri = lo, e = ri * root_stride;
+            const int32_t piece = (int32_t)(it - first[ri]), np_ = pieces[ri];
+            const int32_t v0 = edge_source(g, e);
+            if (np_ == 1) {
+                edge_root(c, g, p, v0, e);
+                continue;
+            }
+            const uint64_t b0 = c->bytes;
+            const int32_t L = edge_root_prefixes(c, g, p, v0, e);
+            if (piece > 0) c->bytes = b0;
+            if (L < 0) continue;
+            c->d2_lo = (int32_t)((int64_t)L * piece / np_);
+            c->d2_hi = (int32_t)((int64_t)L * (piece + 1) / np_);
+            if (p->mode == GM_MODE_CLIQUE) {
+                extern void oracle_clique_dfs(or_ctx*, int);
+                oracle_clique_dfs(c, 2);
+            } else {
+                dfs(c, 2);
+            }
+            c->d2_lo = 0;
+            c->d2_hi = INT32_MAX;
         }
         total += c->sum;
         bytes += c->bytes;
         ctx_free(c);
     }
+    free(first);
+    free(pieces);
     if (alg_bytes) *alg_bytes = bytes;
     return total;
 }
@@ -490,7 +572,12 @@ void oracle_clique_dfs(or_ctx* c, int depth) {
         c->sum += (uint64_t)loop.len;
         return;
     }
-    for (int32_t i = 0; i < loop.len; ++i) {
+    int32_t i0 = 0, i1 = loop.len;
+    if (depth == 2) {
+        i0 = c->d2_lo;
+        if (c->d2_hi < i1) i1 = c->d2_hi;
+    }
+    for (int32_t i = i0; i < i1; ++i) {
         uint32_t v = (uint32_t)loop.ptr[i];
         int64_t l = c->g->vertex[v], r = c->g->vertex[v + 1];
         c->bytes += 12ull;

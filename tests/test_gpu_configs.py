@@ -1,0 +1,127 @@
+"""GPU parity at the benchmark workloads' full sizes.
+
+The expected values were computed by the CPU oracle in the build container
+(tests/golden/make_config_fixtures.py -> tests/golden/config_counts.json);
+here the device regenerates each graph with the same seeded generator and
+must match them bit for bit:
+
+  * 4- and 5-cliques of R-MAT 2^20 ef16 seed 2 and of the Erdos-Renyi control
+    G(2^20, 32/2^20) seed 3: counts and the byte tally, on the degree-oriented
+    graph and on the same orientation renumbered by rank;
+  * house, 5-cycle and 4-clique + pendant on R-MAT 2^22 ef16 seed 4 (analytic
+    schedule) over a fixed root sample e % S == 0: raw sum and byte tally;
+  * the six 4-motifs over the 512-graph batch at root sample e % 256 == 0:
+    every (graph, motif) sum, per-pattern launches and the captured batch;
+  * 6-cycle and 3-prism on R-MAT 2^23 ef32 seed 7 over a fixed root sample.
+"""
+import hashlib
+import json
+import os
+
+import pytest
+
+from graphindor_amd import native as N
+from tests.common import clique_adj, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config_counts.json")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    require_gpu()
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    if not os.path.exists(FIXTURES):
+        pytest.skip("config fixtures not generated")
+    return json.load(open(FIXTURES))
+
+
+def digest(sched):
+    return hashlib.sha256(json.dumps(sched.dump()).encode()).hexdigest()[:16]
+
+
+@pytest.mark.parametrize("key", ["rmat20_s2", "er20_s3"])
+def test_config_cliques_at_full_size(fixtures, key):
+    if key not in fixtures:
+        pytest.skip(f"{key} not in the fixtures")
+    want = fixtures[key]
+    g = N.Graph.rmat(20, 16, 2) if key == "rmat20_s2" else N.Graph.erdos_renyi(1 << 20, 32.0 / (1 << 20), 3)
+    for how in ("degree", "rank"):
+        og = g.orient_by_degree() if how == "degree" else g.orient_by_degree_rank()
+        v, e = (x.copy() for x in og.csr())
+        assert len(e) == want["oriented_edges"]
+        dg = N.DeviceGraph(v, e)
+        for k in (4, 5):
+            print(f"case: {key} {how} k={k}", flush=True)
+            s = N.Schedule.plain(k, clique_adj(k))
+            s.redundancy = 1
+            raw, st = N.count(dg, N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE)), flags=N.GM_COUNT_ALG_BYTES)
+            assert (raw, st.alg_bytes) == (want[f"k{k}"]["count"], want[f"k{k}"]["hybrid1024_bytes"])
+        dg.free()
+
+
+@pytest.fixture(scope="module")
+def rmat22():
+    g = N.Graph.rmat(22, 16, 4)
+    v, e = (x.copy() for x in g.csr())
+    return g.info(), v, e, N.DeviceGraph(v, e)
+
+
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant"])
+def test_config_pm_rmat22_root_sample(fixtures, rmat22, name):
+    if "rmat22_s4" not in fixtures:
+        pytest.skip("rmat22_s4 not in the fixtures")
+    inf, v, e, dg = rmat22
+    want = fixtures["rmat22_s4"]["patterns"][name]
+    assert len(e) == fixtures["rmat22_s4"]["csr_entries"]
+    s, _ = N.select_schedule(5, want["adj"], v, inf["tri_cnt"], None)
+    assert digest(s) == want["schedule"]
+    print(f"case: rmat22 {name} stride {want['stride']}", flush=True)
+    raw, st = N.count_batch(dg, [0, len(e)], [N.DevicePlan(s.plan_desc())], flags=N.GM_COUNT_ALG_BYTES,
+                            root_stride=want["stride"])
+    assert (raw[0][0], st.alg_bytes) == (want["raw"], want["alg_bytes"])
+
+
+def test_config_batch512_motifs(fixtures):
+    if "batch512_s5" not in fixtures:
+        pytest.skip("batch512_s5 not in the fixtures")
+    want = fixtures["batch512_s5"]
+    g, voff, eoff = N.Graph.batch(512, 5, 10.0, 17.0, 4.0, 64.0)
+    inf = g.info()
+    v, e = (x.copy() for x in g.csr())
+    assert len(e) == want["csr_entries"]
+    dg = N.DeviceGraph(v, e)
+    scheds = [N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]) for adj in want["motifs"]]
+    assert [digest(s) for s in scheds] == want["schedules"]
+    plans = [N.DevicePlan(s.plan_desc()) for s in scheds]
+    print("case: batch512 fused launch", flush=True)
+    got, _ = N.count_batch(dg, eoff, plans, root_stride=want["stride"])
+    assert got == want["sums"]
+    print("case: batch512 captured batch", flush=True)
+    b = N.Batch(dg, eoff, plans, root_stride=want["stride"])
+    for _ in range(2):
+        replay, _ = b.run()
+        assert replay == want["sums"]
+    b.free()
+
+
+@pytest.mark.parametrize("name", ["c6", "prism"])
+def test_config_learned_rmat23_root_sample(fixtures, name):
+    if "rmat23_ef32_s7" not in fixtures:
+        pytest.skip("rmat23_ef32_s7 not in the fixtures")
+    want = fixtures["rmat23_ef32_s7"]["patterns"][name]
+    g = N.Graph.rmat(23, 32, 7)
+    inf = g.info()
+    v, e = (x.copy() for x in g.csr())
+    assert len(e) == fixtures["rmat23_ef32_s7"]["csr_entries"]
+    s, _ = N.select_schedule(6, want["adj"], v, inf["tri_cnt"], None)
+    assert digest(s) == want["schedule"]
+    dg = N.DeviceGraph(v, e)
+    print(f"case: rmat23 {name} stride {want['stride']}", flush=True)
+    raw, st = N.count_batch(dg, [0, len(e)], [N.DevicePlan(s.plan_desc())], flags=N.GM_COUNT_ALG_BYTES,
+                            root_stride=want["stride"])
+    assert (raw[0][0], st.alg_bytes) == (want["raw"], want["alg_bytes"])

@@ -292,6 +292,24 @@ def test_edge_rooted_stride_counts_exactly_the_multiples(name):
     assert O.count_edge_rooted_strided(csr, di, 1) == O.count_edge_rooted(csr, di)
 
 
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "c4"])
+def test_split_hub_roots_keep_sums(name):
+    """Roots whose depth-2 loop exceeds 64 candidates are split over threads;
+    the sums equal the unsplit per-root rule (the batch driver) and brute force."""
+    from tests.common import hub_graph
+    n, adj = PATTERNS[name]
+    v, e = hub_graph(400, 7, extra=0.02)
+    csr = O.CSR(v, e)
+    for iep in (True, False):
+        s = schedule_for(csr, n, adj, iep=iep)
+        d = s.plan_desc()
+        got = O.count_edge_rooted_strided(csr, d, 1)[0]
+        assert got == O.count_batch_strided(csr, [0, len(e)], d, 1)[0]
+        assert got // s.redundancy == O.brute_force(csr, n, adj)
+        for S in (3, 17):
+            assert O.count_edge_rooted_strided(csr, d, S)[0] == O.count_batch_strided(csr, [0, len(e)], d, S)[0]
+
+
 @pytest.mark.parametrize("stride", [1, 3, 64])
 def test_batch_strided_per_graph_sums(stride):
     """Per-graph sums of a batch CSR (global ids, graph i owning the root
