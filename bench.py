@@ -25,15 +25,17 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel on the default workload (R-MAT
-# 2^20 ef16 seed 2, degree-oriented), from two rocprofv3 PMC runs (FETCH_SIZE
-# and WRITE_SIZE, each scaled by what it reads for a 1 GiB device copy in the
-# same run); see profiles/clique5_rmat20_hbm_traffic.md.
-TRAFFIC = {(20, "clique_bitmap_big_kernel<5>"): 9.191e9}
+# HBM bytes per launch of the dominant kernel, keyed by (scale, orientation,
+# kernel), from two rocprofv3 PMC runs of this bench (FETCH_SIZE and
+# WRITE_SIZE, each scaled by what it reads for a 1 GiB device copy in the
+# same run).  None where no measurement of the current kernel exists.
+TRAFFIC = {}
 
 
 def log(*a):
@@ -72,26 +74,72 @@ def time_counts(N, dg, plan, steps, warmup):
                 wall_s=mean(walls), kernel_ms=mean(kms), list_ms=mean(lms), bitmap_ms=mean(bms), block_ms=mean(kbs))
 
 
-def cpu_baseline(N, O, v, e, k, target_s=20.0):
-    """Reference-CPU-engine restatement (vertex-rooted, OpenMP dynamic) on the
-    graph the reference's clique_test counts (id-reduced, src/clique_test.cpp:49-76),
-    timed on an even sample of the work: every root, but only its depth-1
-    candidates with index = 1 mod S; the time is scaled by S.  A first pass at
-    S = 4096 sizes S for ~target_s."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    csr = O.CSR(v, e)
-    sched = N.Schedule.build(k, clique_adj(k), len(v) - 1, len(e), 0)
-    desc = sched.plan_desc()
-    t0 = time.perf_counter()
-    O.count_vertex_rooted_strided(csr, desc, threads, 4096, 0)
-    full = (time.perf_counter() - t0) * 4096
-    stride = max(2, min(4096, int(full / target_s)))
-    t0 = time.perf_counter()
-    O.count_vertex_rooted_strided(csr, desc, threads, stride, 1)
-    t = time.perf_counter() - t0
-    return {"value": round(t * stride, 3), "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"{k}-clique on the id-reduced graph (clique_test rule), every root, depth-1 "
-                      f"candidates i = 1 mod {stride}: {t:.1f} s measured, scaled x{stride}"}
+def host_cpu():
+    """CPU model, sockets, cores and threads of this host (lscpu), and the
+    physical core count."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, val = line.partition(":")
+            info[k.strip()] = val.strip()
+    except Exception:
+        pass
+    sockets = int(info.get("Socket(s)", "1") or 1)
+    cores = int(info.get("Core(s) per socket", "0") or 0) * sockets
+    smt = int(info.get("Thread(s) per core", "1") or 1)
+    # a container may see fewer CPUs than the host has cores
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    physical = max(1, min(cores or visible, visible))
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "cores": cores, "threads_per_core": smt,
+            "visible_cpus": visible}, physical
+
+
+def cpu_baseline(N, O, graphs, k, target_s=20.0):
+    """The oracle's restatement of the reference CPU engine (vertex-rooted,
+    OpenMP dynamic; src/graph.cpp:305-566) counting k-cliques, on every
+    physical core with OMP_PROC_BIND=close.  graphs: [(label, vertex, edge)]; the first is the
+    graph the GPU counts (degree-oriented), the second the reference's own
+    id orientation.  Each is counted in full when a first sample says that
+    fits the budget; otherwise on an even sample of the work (every root,
+    depth-1 candidates i = 1 mod S), reported as a partial-progress rate
+    with the extrapolated full time labelled as such."""
+    cpu, threads = host_cpu()
+    out = {"unit": "s", "cores": threads, "kind": "port", "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "cpu": cpu, "graphs": {}}
+    for label, v, e in graphs:
+        csr = O.CSR(v, e)
+        if label == "id_oriented":
+            # clique_test's schedule (src/clique_test.cpp:49-76): its id
+            # restrictions agree with edges kept toward smaller ids
+            sched = N.Schedule.build(k, clique_adj(k), len(v) - 1, len(e), 0)
+        else:
+            # a degree orientation is a DAG in another order: no restrictions
+            sched = N.Schedule.plain(k, clique_adj(k))
+            sched.redundancy = 1
+        desc = sched.plan_desc()
+        t0 = time.perf_counter()
+        O.count_vertex_rooted_strided(csr, desc, threads, 64, 0)
+        est = (time.perf_counter() - t0) * 64
+        if est <= target_s * 1.5:
+            t0 = time.perf_counter()
+            raw = O.count_vertex_rooted(csr, desc, threads)
+            t = time.perf_counter() - t0
+            rec = {"time_s": round(t, 3), "count": int(raw), "measured": "full count, 1 run"}
+        else:
+            stride = max(2, min(4096, int(est / target_s)))
+            t0 = time.perf_counter()
+            O.count_vertex_rooted_strided(csr, desc, threads, stride, 1)
+            t = time.perf_counter() - t0
+            rec = {"time_s": round(t * stride, 3), "measured": f"partial: 1/{stride} of the depth-1 candidates of every "
+                   f"root in {t:.1f} s; time_s extrapolates that rate to the full count"}
+        out["graphs"][label] = rec
+    first = next(iter(out["graphs"].values()))
+    out["value"] = first["time_s"]
+    out["sample"] = (f"{k}-clique, clique_test schedule, on the degree-oriented graph the GPU counts: "
+                     f"{first['measured']}; id-oriented (reference reduce_edges_for_clique) alongside")
+    return out
 
 
 MOTIFS4 = ["0111101111011110", "0111101111001100", "0111101011001000", "0111100010001000", "0110100110010110",
@@ -106,7 +154,9 @@ def batch_leg(N, count=512, stride=256):
     all 6 patterns x 512 graphs in one hipGraph-captured persistent launch
     with the autotuned tuning.  Both count the same root sample (edges e with
     e % stride == 0; the full batch is ~10^2 TB of set work) and must agree
-    on every (graph, pattern) sum."""
+    on every (graph, pattern) sum (and with the oracle's fixture at stride
+    256).  The plain form is reported as wall time (3,072 synchronous calls,
+    as gpu_mc's per-motif launches) and as the sum of its kernel times."""
     t0 = time.time()
     g, voff, eoff = N.Graph.batch(count, 5, 10.0, 17.0, 4.0, 64.0)
     inf = g.info()
@@ -119,11 +169,13 @@ def batch_leg(N, count=512, stride=256):
     # plain: warm once, then time every (graph, pattern) launch
     N.count_batch(dg, eoff[:2], plans[:1], root_stride=stride)
     plain = [[0] * len(plans) for _ in range(count)]
+    plain_kernel_ms = 0.0
     t0 = time.perf_counter()
     for i in range(count):
         for j, p in enumerate(plans):
-            r, _ = N.count_batch(dg, eoff[i:i + 2], [p], root_stride=stride)
+            r, st = N.count_batch(dg, eoff[i:i + 2], [p], root_stride=stride)
             plain[i][j] = r[0][0]
+            plain_kernel_ms += st.kernel_ms
     plain_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     tc = N.autotune_batch(dg, eoff, plans, "motifs4:" + N.degree_class(v), None, root_stride=stride)
@@ -141,85 +193,140 @@ def batch_leg(N, count=512, stride=256):
     batched_s = min(walls)
     log(f"batch: {count} graphs, {len(e)} CSR entries (gen {gen_s:.1f} s), stride {stride}: plain {plain_s:.3f} s, "
         f"batched {batched_s:.3f} s, tune {tune_s:.2f} s {tc}")
+    fix = fixture("batch512_s5")
+    parity = None
+    if fix and fix["stride"] == stride and count == fix["graphs"]:
+        parity = plain == fix["sums"]
+        assert parity, "batch sums differ from the oracle fixture"
     return {"graphs": count, "csr_entries": int(len(e)), "root_sample": f"edges e with e % {stride} == 0",
             "plain_per_pattern_launches_s": round(plain_s, 4), "launches_plain": count * len(plans),
+            "plain_kernel_time_s": round(plain_kernel_ms / 1e3, 4),
             "batched_hipgraph_s": round(batched_s, 4), "speedup": round(plain_s / batched_s, 3),
+            "speedup_kernel_time": round(plain_kernel_ms / 1e3 / batched_s, 3), "oracle_fixture_equal": parity,
             "autotune": {k: tc[k] for k in ("waves_per_cu", "chunk_edges", "lane_tail", "tried")},
             "autotune_s": round(tune_s, 3), "sums_equal": True}
 
 
 PM5 = [("house", "0101110101010101010011000"), ("c5", "0100110100010100010110010"),
        ("k4_pendant", "0111110110110101110010000")]
+PM6 = [("c6", "010001101000010100001010000101100010"), ("prism", "011100101010110001100011010101001110")]
 MODEL_PATH = os.path.join(ROOT, "graphindor_amd", "data", "cost_model.gscm")
+FIXTURES = os.path.join(ROOT, "tests", "golden", "config_counts.json")
+# Root samples e % S == 0 of the pattern legs: those of the parity fixtures.
+PM_STRIDES = {"house": 4096, "c5": 4194304, "k4_pendant": 256, "c6": 4194304, "prism": 1048576}
+KNOBS = [{"lane_tail": 0, "hub_degree": 0}, {"lane_tail": -1, "hub_degree": 0}, {"lane_tail": 0, "hub_degree": -1},
+         {"lane_tail": -1, "hub_degree": -1}]
 
 
-def pm_leg(N, scale=22, seed=4, budget_s=8.0):
-    """5-vertex pattern matching with IEP on R-MAT 2^scale ef16 (house, 5-cycle,
-    4-clique + pendant): the plain analytic schedule and, when a trained cost
-    model is present, the learned pick.  A full count on R-MAT 2^22 is far
-    beyond a bench run (hub neighbourhoods of 10^5 vertices), so each pattern
-    is timed on the root sample e % S == 0, S halving from 2^16 until the
-    analytic run takes about budget_s / 8 (or doubling up to 2^20 while it
-    exceeds budget_s); both schedules use the same S.  The analytic schedule
-    is also timed with the lane-parallel tail off (`analytic_tail_off_ms`):
-    with depth-2 slice donation that is faster on several patterns.
-    Sampled sums differ between schedules (a root edge maps to different
-    embeddings), so the comparison is of times; full-count parity is in the
-    GPU tests."""
-    t0 = time.time()
-    g = N.Graph.rmat(scale, 16, seed)
+def fixture(key):
+    if not os.path.exists(FIXTURES):
+        return None
+    return json.load(open(FIXTURES)).get(key)
+
+
+def pm_patterns(N, g, patterns, n, fix, budget_ms=60000):
+    """Each pattern on its fixed root sample: the plain analytic schedule with
+    the library defaults (its raw sum and byte tally checked against the
+    oracle's fixture), the four settings of the lane-parallel last level and
+    the hub bitmaps (autotuning, on the same sample), and the schedule the
+    learned cost model proposes and a measurement on the same sample
+    confirms."""
     inf = g.info()
-    v, e = g.csr()
-    v, e = v.copy(), e.copy()
-    gen_s = time.time() - t0
+    v, e = (x.copy() for x in g.csr())
     dg = N.DeviceGraph(v, e)
     model = N.CostModel.load(MODEL_PATH) if os.path.exists(MODEL_PATH) else None
-    out = {"graph": f"R-MAT 2^{scale} ef16 seed {seed}", "csr_entries": int(len(e)),
-           "max_degree": int((v[1:] - v[:-1]).max()), "model": os.path.basename(MODEL_PATH) if model else None,
-           "patterns": {}}
-    for name, adj in PM5:
-        row = {}
-        sa, ia = N.select_schedule(5, adj, v, inf["tri_cnt"], None)
-        pa = N.DevicePlan(sa.plan_desc())
-        limit = int(budget_s * 1000)
-        S = 1 << 16
+    out = {}
+    for name, adj in patterns:
+        S = PM_STRIDES[name]
+        row = {"root_stride": S, "sampled_roots": (len(e) + S - 1) // S}
 
-        def timed(plan, **tuning):
+        def timed(plan, flags=0, **tun):
             try:
-                return N.count_batch(dg, [0, len(e)], [plan], root_stride=S, time_limit_ms=limit,
-                                     **tuning)[1].kernel_ms
+                raw, st = N.count_batch(dg, [0, len(e)], [plan], flags=flags, root_stride=S, time_limit_ms=budget_ms,
+                                        **tun)
+                return raw[0][0], st
             except N.GmTimeout:
-                return None
-        t = timed(pa)
-        while t is None and S < (1 << 20):
-            S *= 4
-            t = timed(pa)
-        grown = S > (1 << 16)
-        while not grown and t is not None and t / 1e3 < budget_s / 8 and S > 1:
-            S //= 2
-            t = timed(pa)
-        row.update({"root_stride": S, "analytic_candidate": ia["analytic"], "candidates": ia["candidates"],
-                    "select_ms_analytic": round(ia["select_ms"], 2)})
-        if t is None:
-            # a few hub roots of the smallest sample already exceed the budget
-            row["analytic_ms"] = f"over {limit} ms"
-            out["patterns"][name] = row
-            log(f"pm {name}: stride {S}: over the {limit} ms budget")
+                return None, None
+        sa, ia = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
+        pa = N.DevicePlan(sa.plan_desc())
+        raw, st = timed(pa, N.GM_COUNT_ALG_BYTES)
+        if raw is None:
+            row["plain_ms"] = f"over {budget_ms} ms"
+            out[name] = row
             continue
-        t = timed(pa)
-        row["analytic_ms"] = round(t, 3) if t is not None else f"over {limit} ms"
-        t_off = timed(pa, lane_tail=-1)
-        row["analytic_tail_off_ms"] = round(t_off, 3) if t_off is not None else f"over {limit} ms"
-        if model is not None and t is not None:
-            sl, il = N.select_schedule_measured(5, adj, g, model)
-            t2 = timed(N.DevicePlan(sl.plan_desc()))
-            row.update({"learned_ms": round(t2, 3) if t2 is not None else f"over {limit} ms",
-                        "learned_candidate": il["chosen"], "select_ms_learned": round(il["select_ms"], 2)})
-            if t2 is not None:
-                row["learned_speedup"] = round(t / max(t2, 1e-6), 3)
-        out["patterns"][name] = row
-        log(f"pm {name}: stride {S}: {row}")
-    out["gen_s"] = round(gen_s, 1)
+        f = (fix or {}).get("patterns", {}).get(name)
+        if f and f["stride"] == S:
+            row["parity"] = {"raw": raw, "alg_bytes": st.alg_bytes, "oracle_equal": (raw, st.alg_bytes) == (f["raw"], f["alg_bytes"])}
+            assert row["parity"]["oracle_equal"], f"{name}: device differs from the oracle fixture"
+        times = {}
+        for kn in KNOBS:
+            r2, st2 = timed(pa, **kn)
+            assert r2 == raw
+            times[f"tail{'off' if kn['lane_tail'] < 0 else 'on'}_hub{'off' if kn['hub_degree'] < 0 else 'on'}"] = \
+                round(st2.kernel_ms, 3)
+        row["plain_ms"] = times["tailon_hubon"]  # library defaults
+        row["analytic_knobs_ms"] = times
+        best_knob = min(KNOBS, key=lambda kn: times[f"tail{'off' if kn['lane_tail'] < 0 else 'on'}_hub{'off' if kn['hub_degree'] < 0 else 'on'}"])
+        row["analytic_tuned_ms"] = min(times.values())
+        row["analytic_candidate"] = ia["analytic"]
+        row["candidates"] = ia["candidates"]
+        if model is not None:
+            sl, il = N.select_schedule_measured(n, adj, g, model, root_stride=S,
+                                                time_limit_ms=int(max(200, 3 * row["plain_ms"])))
+            pl = N.DevicePlan(sl.plan_desc())
+            r3, st3 = timed(pl)
+            r4, st4 = timed(pl, **best_knob)
+            row.update({"learned_candidate": il["chosen"], "select_ms_learned": round(il["select_ms"], 1),
+                        "learned_ms": round(st3.kernel_ms, 3) if st3 else f"over {budget_ms} ms",
+                        "learned_tuned_ms": round(st4.kernel_ms, 3) if st4 else f"over {budget_ms} ms"})
+            if st4:
+                row["learned_tuned_speedup_vs_plain"] = round(row["plain_ms"] / max(st4.kernel_ms, 1e-6), 3)
+        out[name] = row
+        log(f"pm {name}: {row}")
+    dg.free()
+    return out
+
+
+def pm_leg(N, scale=22, seed=4):
+    """5-vertex pattern matching with IEP on R-MAT 2^scale ef16 seed 4 (house,
+    5-cycle, 4-clique + pendant), each on the fixed root sample of its parity
+    fixture.  Full counts at this size are beyond a bench run (the oracle's
+    byte model puts the 5-cycle near 3e18 bytes of set work); the 4-clique +
+    pendant full count is timed by --pm-full."""
+    t0 = time.time()
+    g = N.Graph.rmat(scale, 16, seed)
+    gen_s = time.time() - t0
+    fix = fixture(f"rmat{scale}_s{seed}")
+    out = {"graph": f"R-MAT 2^{scale} ef16 seed {seed}", "csr_entries": int(g.info()["e_cnt"]), "gen_s": round(gen_s, 1),
+           "patterns": pm_patterns(N, g, PM5, 5, fix)}
+    return out
+
+
+def pm6_leg(N):
+    """The learned-model evaluation workload: 6-cycle and 3-prism on R-MAT
+    2^23 ef32 seed 7, each on the fixed root sample of its parity fixture."""
+    t0 = time.time()
+    g = N.Graph.rmat(23, 32, 7)
+    gen_s = time.time() - t0
+    fix = fixture("rmat23_ef32_s7")
+    return {"graph": "R-MAT 2^23 ef32 seed 7", "csr_entries": int(g.info()["e_cnt"]), "gen_s": round(gen_s, 1),
+            "patterns": pm_patterns(N, g, PM6, 6, fix)}
+
+
+def pm_full(N, scale=22, seed=4):
+    """4-clique + pendant on R-MAT 2^22 counted in full: plain defaults and
+    the autotuned settings, the count divided by the IEP redundancy."""
+    g = N.Graph.rmat(scale, 16, seed)
+    inf = g.info()
+    v, e = (x.copy() for x in g.csr())
+    dg = N.DeviceGraph(v, e)
+    s, _ = N.select_schedule(5, dict(PM5)["k4_pendant"], v, inf["tri_cnt"], None)
+    p = N.DevicePlan(s.plan_desc())
+    out = {}
+    for label, kn in (("plain", {}), ("tail_off", {"lane_tail": -1})):
+        raw, st = N.count(dg, p, **kn)
+        out[label] = {"count": raw // s.redundancy, "raw": raw, "s": round(st.kernel_ms / 1e3, 3)}
+        log(f"k4_pendant full count ({label}): {out[label]}")
     return out
 
 
@@ -241,17 +348,46 @@ def calibration_copy(mb):
     log(f"calibration copy: {mb} MiB device to device")
 
 
+def intersection_leg(N, dg, v, e):
+    """The intersection kernel in isolation, on the operand pairs of the real
+    workload: every depth-1 pair N+(v0) ∩ N+(v1) of the oriented graph -- one
+    per oriented edge -- intersected for its size (their sum is the triangle
+    count).  Bytes by
+    the size-only model, 4 (|A| + |B|) per pair.  Timed with the bitmap
+    engine (rows of N+(v0) in LDS, every N+(v1) streamed once) and with the
+    list engine (lane-parallel binary searches, bitmap engine off)."""
+    deg = np.diff(v).astype(np.int64)
+    pair_bytes = 4 * int((deg * deg).sum() + deg[e].sum())
+    s = N.Schedule.plain(3, clique_adj(3))
+    s.redundancy = 1
+    plan = N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE))
+    out = {"pairs": int(len(e)), "alg_bytes": pair_bytes}
+    for label, kn in (("bitmap_engine", {}), ("list_engine", {"bitmap_max_degree": -1})):
+        raw, _ = N.count(dg, plan, **kn)
+        best = min(N.count(dg, plan, **kn)[1].kernel_ms for _ in range(3))
+        gbs = pair_bytes / (best / 1e3) / 1e9
+        out[label] = {"triangles": int(raw), "ms": round(best, 3), "achieved_GBs": round(gbs, 1),
+                      "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)}
+        log(f"intersection ({label}): {len(e)} pairs, {pair_bytes / 1e9:.2f} GB in {best:.3f} ms = {gbs:.0f} GB/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--orientation", choices=("rank", "degree"), default="rank",
+                    help="rank: degree orientation with vertices renumbered by (degree, id); degree: as "
+                         "degree_orientation_init leaves them")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-control", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched motif leg")
-    ap.add_argument("--batch-stride", type=int, default=256)
+    ap.add_argument("--batch-stride", type=int, default=64)
     ap.add_argument("--no-pm", action="store_true", help="skip the 5-vertex pattern-matching leg")
+    ap.add_argument("--no-pm6", action="store_true", help="skip the 6-vertex (learned model) leg")
+    ap.add_argument("--pm-full", action="store_true", help="also count 4-clique + pendant on R-MAT 2^22 in full")
     ap.add_argument("--pm-scale", type=int, default=22)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC traffic calibration)")
@@ -261,6 +397,9 @@ def main():
               "(the counting path runs on one GPU; several would each take a range of the root edges)", file=sys.stderr)
         sys.exit(2)
 
+    # The CPU baseline's OpenMP threads stay on neighbouring cores (read by
+    # libgomp when the host library first loads).
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from graphindor_amd import native as N
     if N.device_count() < 1:
         sys.exit("no GPU visible")
@@ -272,12 +411,12 @@ def main():
     g = N.Graph.rmat(args.scale, 16, 2)
     info = g.info()
     t1 = time.time()
-    og = g.orient_by_degree()
+    og = g.orient_by_degree_rank() if args.orientation == "rank" else g.orient_by_degree()
     orient_s = time.time() - t1
     v, e = og.csr()
     v, e = v.copy(), e.copy()
     log(f"graph: R-MAT 2^{args.scale} ef16 seed 2: {info['v_cnt']} vertices, {info['e_cnt']} CSR entries, "
-        f"{len(e)} after degree orientation ({time.time() - t0:.1f} s, orientation {orient_s:.2f} s)")
+        f"{len(e)} after {args.orientation} orientation ({time.time() - t0:.1f} s, orientation {orient_s:.2f} s)")
     dg = N.DeviceGraph(v, e)
 
     results = {}
@@ -291,10 +430,29 @@ def main():
         log(f"{k}-clique: {r['count']} in {r['wall_s']:.4f} s (region {r['kernel_ms']:.2f} ms: bitmap engine "
             f"{r['bitmap_ms']:.2f} ms of which block-per-root {r['block_ms']:.2f} ms, list engine {r['list_ms']:.2f} ms), "
             f"alg bytes {r['alg_bytes']} (bitmap {r['bitmap_alg_bytes']}, block-per-root {r['block_alg_bytes']})")
+    fix = fixture(f"rmat{args.scale}_s2")
+    if fix:
+        for k in (4, 5):
+            assert (results[k]["count"], results[k]["alg_bytes"]) == \
+                (fix[f"k{k}"]["count"], fix[f"k{k}"]["hybrid1024_bytes"]), f"{k}-clique differs from the oracle fixture"
+    isect = intersection_leg(N, dg, v, e)
 
-    # The reference's own orientation (edges kept toward smaller ids,
-    # src/graph.cpp:710-735), 4-cliques only: its 5-clique count is far
-    # slower and the count is the same.
+    # The same counts on the orientation degree_orientation_init leaves (ids
+    # not renumbered) and on the reference's own (edges kept toward smaller
+    # ids, src/graph.cpp:710-735; 4-cliques only, its 5-clique count is far
+    # slower).  Counts are identical.
+    other = {}
+    og2 = g.orient_by_degree() if args.orientation == "rank" else g.orient_by_degree_rank()
+    dv, de = (x.copy() for x in og2.csr())
+    del og2
+    dgo = N.DeviceGraph(dv, de)
+    for k in (4, 5):
+        s = N.Schedule.plain(k, clique_adj(k))
+        s.redundancy = 1
+        ro = time_counts(N, dgo, N.DevicePlan(s.plan_desc(N.GM_MODE_CLIQUE)), 1, 1)
+        assert ro["count"] == results[k]["count"]
+        other[f"time_{k}clique_{'degree' if args.orientation == 'rank' else 'rank'}_orientation_s"] = round(ro["wall_s"], 6)
+    dgo.free()
     g.reduce_for_clique()
     iv, ie = g.csr()
     dgi = N.DeviceGraph(iv.copy(), ie.copy())
@@ -303,13 +461,13 @@ def main():
     ri = time_counts(N, dgi, N.DevicePlan(s4.plan_desc(N.GM_MODE_CLIQUE)), 1, 1)
     raw_i, wall_i = ri["count"], ri["wall_s"]
     assert raw_i == results[4]["count"], "orientation changed the 4-clique count"
-    log(f"id-oriented 4-clique: {raw_i} in {wall_i:.4f} s")
-    del dgi
+    log(f"id-oriented 4-clique: {raw_i} in {wall_i:.4f} s; other orientation: {other}")
+    dgi.free()
 
     control = None
     if not args.no_control:
         ge = N.Graph.erdos_renyi(1 << args.scale, 32.0 / (1 << args.scale), 3)
-        ge = ge.orient_by_degree()
+        ge = ge.orient_by_degree_rank()
         ve, ee = ge.csr()
         dge = N.DeviceGraph(ve.copy(), ee.copy())
         control = {}
@@ -320,12 +478,13 @@ def main():
             rc = time_counts(N, dge, plan, 1, 1)
             control[f"k{k}"] = {"count": rc["count"], "time_s": round(rc["wall_s"], 6)}
             log(f"ER control {k}-clique: {rc['count']} in {rc['wall_s']:.4f} s")
+        dge.free()
 
     r5 = results[5]
     # Dominant kernel of the 5-clique count: the one with the larger share of
     # the region; its algorithmic bytes over its own HIP-event time.
     parts = [
-        {"kernel": "clique_bitmap_kernel<16,5>", "ms": r5["bitmap_ms"] - r5["block_ms"],
+        {"kernel": "clique_bitmap_small_kernel<5>", "ms": r5["bitmap_ms"] - r5["block_ms"],
          "bytes": r5["bitmap_alg_bytes"] - r5["block_alg_bytes"]},
         {"kernel": "clique_bitmap_big_kernel<5>", "ms": r5["block_ms"], "bytes": r5["block_alg_bytes"]},
         {"kernel": "enumerate_kernel<1,5>", "ms": r5["list_ms"], "bytes": r5["alg_bytes"] - r5["bitmap_alg_bytes"]},
@@ -349,14 +508,18 @@ def main():
                                "(a,b,c,d)=(0.57,0.19,0.19,0.05), plain schedule",
                    "vertices": info["v_cnt"], "csr_entries": info["e_cnt"], "reduced_edges": int(len(e)),
                    "count_5clique": r5["count"], "count_4clique": results[4]["count"],
+                   "oracle_fixture_equal": bool(fix),
                    "time_4clique_s": round(results[4]["wall_s"], 6),
-                   "orientation": "degree (reference degree_orientation_init), host preprocessing "
-                                  f"{orient_s:.2f} s outside the timed region",
+                   "orientation": (f"{args.orientation}: degree orientation (reference degree_orientation_init)"
+                                   + (" with vertices renumbered by (degree, id) rank" if args.orientation == "rank" else "")
+                                   + f", host preprocessing {orient_s:.2f} s outside the timed region"),
+                   **other,
                    "time_4clique_id_orientation_s": round(wall_i, 6),
-                   "er_control": control},
+                   "er_control": control,
+                   "intersection": isect},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC.get((args.scale, dom["kernel"])),
+                     "traffic": TRAFFIC.get((args.scale, args.orientation, dom["kernel"])),
                      "kernel": dom["kernel"], "kernel_ms": round(dom["ms"], 3), "alg_bytes": dom["bytes"],
                      "region_ms": round(r5["kernel_ms"], 3),
                      "kernels_ms": {x["kernel"]: round(x["ms"], 3) for x in parts}},
@@ -365,9 +528,13 @@ def main():
         out["config"]["batch_4motif"] = batch_leg(N, 512, args.batch_stride)
     if not args.no_pm:
         out["config"]["pm_5vertex"] = pm_leg(N, args.pm_scale, 4)
+    if not args.no_pm6:
+        out["config"]["pm_6vertex_learned"] = pm6_leg(N)
+    if args.pm_full:
+        out["config"]["pm_full_k4_pendant"] = pm_full(N, args.pm_scale, 4)
     if not args.no_cpu:
         from oracle import oracle as O
-        out["cpu_baseline"] = cpu_baseline(N, O, iv.copy(), ie.copy(), 5)
+        out["cpu_baseline"] = cpu_baseline(N, O, [("degree_oriented", v, e), ("id_oriented", iv.copy(), ie.copy())], 5)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

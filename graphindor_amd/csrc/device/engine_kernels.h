@@ -34,6 +34,7 @@ struct DevPlan {
 
 struct SetRef {
     const uint32_t* ptr;
+    const unsigned long long* bm;  // the set as a bitmap over vertex ids (a hub's neighbour list), or null
     int32_t len;
     int32_t pad;
 };
@@ -49,6 +50,7 @@ constexpr int kTailMaxIep = 8;
 struct TailCtx {
     int32_t pid[kTailMaxPrefix];
     const uint32_t* fptr[kTailMaxPrefix];
+    const unsigned long long* fbm[kTailMaxPrefix];  // the father's hub bitmap, or null
     int32_t flen[kTailMaxPrefix];
     bool alias[kTailMaxPrefix];
     int32_t base[kTailMaxIep];
@@ -131,6 +133,10 @@ struct KernelArgs {
     int32_t slice_len;        // depth-2 candidates per donated slice
     int32_t eager_donation;   // 1: donate long remainders at once, not only when roots run out
     int64_t time_limit_ticks; // wall-clock ticks a wave may run (0: no limit)
+    const unsigned long long* hub_bits;  // neighbour bitmaps of the hub vertices (null: none)
+    const int32_t* hub_slot;             // per vertex: index of its bitmap, or -1
+    int64_t hub_words;                   // 64-bit words per bitmap (ceil(nv / 64))
+    int32_t stage_cap;                   // elements of tail fathers staged in LDS (<= kStage)
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -185,6 +191,24 @@ __device__ __forceinline__ int32_t lower_bound_u(const uint32_t* b, int32_t nb, 
     return lo;
 }
 
+// Hub vertices (degree above a threshold) carry their neighbour set as a
+// bitmap over vertex ids: membership in a hub's list is one load instead of
+// a binary search of log2(degree) dependent loads.
+__device__ __forceinline__ const unsigned long long* hub_bitmap(const KernelArgs& a, uint32_t v) {
+    if (a.hub_bits == nullptr) return nullptr;
+    const int32_t slot = a.hub_slot[v];
+    return slot < 0 ? nullptr : a.hub_bits + (int64_t)slot * a.hub_words;
+}
+
+__device__ __forceinline__ bool bit_test(const unsigned long long* bm, uint32_t x) {
+    return ((bm[x >> 6] >> (x & 63u)) & 1ull) != 0ull;
+}
+
+// x in the sorted list b[0:nb), by its bitmap when it has one.
+__device__ __forceinline__ bool member_of(const uint32_t* b, int32_t nb, const unsigned long long* bm, uint32_t x) {
+    return bm ? bit_test(bm, x) : contains(b, nb, x);
+}
+
 struct Wave {
     KernelArgs a;
     const DevPlan* pl;
@@ -213,6 +237,7 @@ __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
     SetRef s = w.lds->sets[pid];
     SetRef r;
     r.ptr = (const uint32_t*)uni64((int64_t)s.ptr);
+    r.bm = (const unsigned long long*)uni64((int64_t)s.bm);
     r.len = unii(s.len);
     r.pad = 0;
     return r;
@@ -242,9 +267,11 @@ __device__ __forceinline__ bool out_of_time(Wave& w) {
     return w.aborted != 0;
 }
 
-__device__ __forceinline__ void put_set(Wave& w, int pid, const uint32_t* ptr, int32_t len) {
+__device__ __forceinline__ void put_set(Wave& w, int pid, const uint32_t* ptr, int32_t len,
+                                        const unsigned long long* bm = nullptr) {
     if (lane_id() == 0) {
         w.lds->sets[pid].ptr = ptr;
+        w.lds->sets[pid].bm = bm;
         w.lds->sets[pid].len = len;
     }
     wave_sync();
@@ -267,8 +294,12 @@ __device__ __forceinline__ bool in_emb(const Wave& w, uint32_t v) {
 }
 
 // Wave-cooperative |a ∩ b|, compacted into out (out == nullptr: size only).
-__device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* out,
-                             int32_t cap) {
+// Lane i takes element i of the shorter list and looks it up in the longer
+// one: one bit test when the longer is a hub list with a bitmap (bma / bmb),
+// else a branchless binary search.
+__device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t na, const unsigned long long* bma,
+                                             const uint32_t* b, int32_t nb, const unsigned long long* bmb,
+                                             uint32_t* out, int32_t cap) {
     if (na > nb) {
         const uint32_t* t = a;
         a = b;
@@ -276,6 +307,9 @@ __device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t
         int32_t tn = na;
         na = nb;
         nb = tn;
+        const unsigned long long* tb = bma;
+        bma = bmb;
+        bmb = tb;
     }
     int32_t cnt = 0;
     if (na == 0) return 0;
@@ -286,7 +320,7 @@ __device__ __forceinline__ int32_t intersect(Wave& w, const uint32_t* a, int32_t
         uint32_t x = 0;
         if (i < na) {
             x = a[i];
-            hit = contains(b, nb, x);
+            hit = member_of(b, nb, bmb, x);
         }
         uint64_t m = ballot(hit);
         if (out != nullptr && hit) {
@@ -311,22 +345,25 @@ __device__ __forceinline__ int32_t subtract_emb(const Wave& w, const uint32_t* s
     return lim - (int32_t)__popcll(ballot(hit));
 }
 
-__device__ __forceinline__ void build(Wave& w, int pid, const uint32_t* in, int32_t nin) {
+// Prefix pid for the vertex just fixed, whose neighbour list is in[0:nin)
+// (bm_in: its bitmap if it is a hub).
+__device__ __forceinline__ void build(Wave& w, int pid, const uint32_t* in, int32_t nin,
+                                      const unsigned long long* bm_in) {
     const DevPlan& p = *w.pl;
     int f = p.father[pid];
     if (f < 0) {
-        put_set(w, pid, in, nin);
+        put_set(w, pid, in, nin, bm_in);
         return;
     }
     SetRef fs = get_set(w, f);
     if (p.size_only[pid]) {
-        int32_t c = intersect(w, fs.ptr, fs.len, in, nin, nullptr, 0);
+        int32_t c = intersect(w, fs.ptr, fs.len, fs.bm, in, nin, bm_in, nullptr, 0);
         put_set(w, pid, nullptr, c);
         if (w.a.stats) w.bytes += 4ull * (unsigned long long)(fs.len + nin);
         return;
     }
     uint32_t* out = w.scratch + (int64_t)pid * w.a.cap;
-    int32_t c = intersect(w, fs.ptr, fs.len, in, nin, out, w.a.cap);
+    int32_t c = intersect(w, fs.ptr, fs.len, fs.bm, in, nin, bm_in, out, w.a.cap);
     put_set(w, pid, out, c < w.a.cap ? c : w.a.cap);  // overflow is reported, never read past
     if (w.a.stats) w.bytes += 4ull * (unsigned long long)(fs.len + nin + c);
 }
@@ -377,6 +414,7 @@ __device__ __forceinline__ SetRef remove_anti_edge(Wave& w, SetRef in) {
     wave_sync();
     SetRef r;
     r.ptr = out;
+    r.bm = nullptr;
     r.len = cnt < w.a.cap_vi ? cnt : w.a.cap_vi;
     r.pad = 0;
     return r;
@@ -478,21 +516,23 @@ __device__ __forceinline__ void clique_tail_count(Wave& w, SetRef L) {
 // ---------------------------------------------------------------------------
 constexpr int32_t kLaneTailWork = 64;  // default light/heavy bound (gm_tuning.lane_tail > 0 sets it)
 
-// |F ∩ N| over the elements below `limit`, iterating the shorter list (per lane).
-__device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
+// |F ∩ N| over the elements below `limit`, iterating the shorter list (per
+// lane) and looking its elements up in the other (by bitmap when it has one).
+__device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t nF, const unsigned long long* fbm,
+                                                    const uint32_t* Nv, int32_t nN, const unsigned long long* nbm,
                                                     uint32_t limit) {
     int32_t c = 0;
     if (nF <= nN) {
         for (int32_t t = 0; t < nF; ++t) {
             uint32_t x = F[t];
             if (x >= limit) break;
-            c += contains(Nv, nN, x) ? 1 : 0;
+            c += member_of(Nv, nN, nbm, x) ? 1 : 0;
         }
     } else {
         for (int32_t t = 0; t < nN; ++t) {
             uint32_t x = Nv[t];
             if (x >= limit) break;
-            c += contains(F, nF, x) ? 1 : 0;
+            c += member_of(F, nF, fbm, x) ? 1 : 0;
         }
     }
     return c;
@@ -500,14 +540,17 @@ __device__ __forceinline__ int32_t lane_isect_count(const uint32_t* F, int32_t n
 
 // The same count by the whole wave (uniform arguments and result): lane i
 // takes element i of each 64-element step of the shorter list.
-__device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t nF, const uint32_t* Nv, int32_t nN,
+__device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t nF, const unsigned long long* fbm,
+                                                    const uint32_t* Nv, int32_t nN, const unsigned long long* nbm,
                                                     uint32_t limit) {
     const uint32_t* s = F;
     const uint32_t* l = Nv;
+    const unsigned long long* lbm = nbm;
     int32_t ns = nF, nl = nN;
     if (nF > nN) {
         s = Nv;
         l = F;
+        lbm = fbm;
         ns = nN;
         nl = nF;
     }
@@ -515,7 +558,7 @@ __device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t n
     int32_t c = 0;
     for (int32_t b = 0; b < lim; b += kWave) {
         const int32_t i = b + lane_id();
-        const bool hit = i < lim && contains(l, nl, s[i]);
+        const bool hit = i < lim && member_of(l, nl, lbm, s[i]);
         c += (int32_t)__popcll(ballot(hit));
     }
     return c;
@@ -528,11 +571,13 @@ __device__ __forceinline__ int32_t wave_isect_count(const uint32_t* F, int32_t n
 // lane evaluates its own v.
 template <int D, bool WAVE>
 __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, uint32_t v, const uint32_t* Nv,
-                                               int32_t dv, unsigned long long& acc, unsigned long long& bytes) {
+                                               int32_t dv, const unsigned long long* vbm, unsigned long long& acc,
+                                               unsigned long long& bytes) {
     const DevPlan& p = *w.pl;
-    auto count = [&](const uint32_t* F, int32_t nF, const uint32_t* N2, int32_t n2, uint32_t limit) -> int32_t {
-        if constexpr (WAVE) return wave_isect_count(F, nF, N2, n2, limit);
-        else return lane_isect_count(F, nF, N2, n2, limit);
+    auto count = [&](const uint32_t* F, int32_t nF, const unsigned long long* fbm, const uint32_t* N2, int32_t n2,
+                     uint32_t limit) -> int32_t {
+        if constexpr (WAVE) return wave_isect_count(F, nF, fbm, N2, n2, vbm, limit);
+        else return lane_isect_count(F, nF, fbm, N2, n2, vbm, limit);
     };
     int32_t sz[kTailMaxPrefix];
 #pragma unroll
@@ -542,18 +587,18 @@ __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, 
         if (c.alias[j]) {
             sz[j] = dv;
         } else {
-            sz[j] = count(c.fptr[j], c.flen[j], Nv, dv, 0xffffffffu);
+            sz[j] = count(c.fptr[j], c.flen[j], c.fbm[j], Nv, dv, 0xffffffffu);
             bytes += 4ull * (unsigned long long)(c.flen[j] + dv);
         }
         if (sz[j] == p.brk[c.pid[j]]) return;
     }
     // x in S_j  <=>  x in N(v) and (alias or x in F_j).
     auto member = [&](int j, uint32_t x) -> bool {
-        bool in_n = contains(Nv, dv, x);
+        bool in_n = member_of(Nv, dv, vbm, x);
         bool in_f = true;
 #pragma unroll
         for (int jj = 0; jj < kTailMaxPrefix; ++jj)
-            if (jj == j && !c.alias[jj]) in_f = contains(c.fptr[jj], c.flen[jj], x);
+            if (jj == j && !c.alias[jj]) in_f = member_of(c.fptr[jj], c.flen[jj], c.fbm[jj], x);
         return in_n && in_f;
     };
     if (p.k >= 2) {
@@ -611,15 +656,17 @@ __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, 
         if (c.qat >= 0) {
             bool al = false;
             const uint32_t* F = nullptr;
+            const unsigned long long* Fbm = nullptr;
             int32_t nF = 0;
 #pragma unroll
             for (int jj = 0; jj < kTailMaxPrefix; ++jj)
                 if (jj == c.qat) {
                     al = c.alias[jj];
                     F = c.fptr[jj];
+                    Fbm = c.fbm[jj];
                     nF = c.flen[jj];
                 }
-            res = al ? lower_bound_u(Nv, dv, b1) : count(F, nF, Nv, dv, b1);
+            res = al ? lower_bound_u(Nv, dv, b1) : count(F, nF, Fbm, Nv, dv, b1);
 #pragma unroll
             for (int x = 0; x < D; ++x)
                 if (w.emb[x] < b1 && member(c.qat, w.emb[x])) --res;
@@ -655,6 +702,7 @@ __device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
         c.pid[j] = q;
         c.alias[j] = true;
         c.fptr[j] = nullptr;
+        c.fbm[j] = nullptr;
         c.flen[j] = 0;
         if (q != -1) {
             int f = p.father[q];
@@ -662,13 +710,14 @@ __device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
                 SetRef fs = get_set(w, f);
                 c.alias[j] = false;
                 c.fptr[j] = fs.ptr;
+                c.fbm[j] = fs.bm;
                 c.flen[j] = fs.len;
                 staged += fs.len;
             }
             q = p.next[q];
         }
     }
-    if (staged <= kStage) {
+    if (staged <= w.a.stage_cap) {
         int32_t off = 0;
 #pragma unroll
         for (int j = 0; j < kTailMaxPrefix; ++j) {
@@ -702,6 +751,7 @@ __device__ __forceinline__ void pattern_tail(Wave& w, SetRef loop) {
     // Last loop set when there is no IEP tail.
     c.qat = -1;
     c.qset.ptr = nullptr;
+    c.qset.bm = nullptr;
     c.qset.len = 0;
     c.qset.pad = 0;
     if (p.k == 0) {
@@ -750,7 +800,7 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
                 for (int j = 0; j < kTailMaxPrefix; ++j)
                     if (c.pid[j] != -1 && !c.alias[j]) work += c.flen[j] < dv ? c.flen[j] : dv;
                 heavy = work > w.a.tail_work;
-                if (!heavy) tail_candidate<D, false>(w, c, v, w.a.col + l, dv, acc, bytes);
+                if (!heavy) tail_candidate<D, false>(w, c, v, w.a.col + l, dv, hub_bitmap(w.a, v), acc, bytes);
             }
         }
         // Heavy candidates, one at a time, wave-wide.
@@ -761,7 +811,7 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
             const uint32_t hv = rdlane(v, j);
             const int64_t l = w.a.row[hv], r = w.a.row[hv + 1];
             unsigned long long a2 = 0, b2 = 0;
-            tail_candidate<D, true>(w, c, hv, w.a.col + l, (int32_t)(r - l), a2, b2);
+            tail_candidate<D, true>(w, c, hv, w.a.col + l, (int32_t)(r - l), hub_bitmap(w.a, hv), a2, b2);
             if (ln == 0) {
                 acc += a2;
                 bytes += b2;
@@ -902,9 +952,10 @@ __device__ __forceinline__ void dfs(Wave& w) {
                 if (MODE == GM_MODE_PATTERN && in_emb<D>(w, v)) continue;
                 int64_t l = rdlane64(rb, j), r = rdlane64(re, j);
                 if (w.a.stats) w.bytes += 8ull;
+                const unsigned long long* vbm = hub_bitmap(w.a, v);
                 bool prune = false;
                 for (int pid = p.last[D]; pid != -1; pid = p.next[pid]) {
-                    build(w, pid, w.a.col + l, (int32_t)(r - l));
+                    build(w, pid, w.a.col + l, (int32_t)(r - l), vbm);
                     if (get_set(w, pid).len == p.brk[pid]) {
                         prune = true;
                         break;
@@ -941,10 +992,12 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
     if (donated) w.a.stats = 0;
     const uint64_t t_build = w.a.queue ? (uint64_t)uni64((int64_t)wall_clock64()) : 0ull;
     int64_t l0 = w.a.row[v0], r0 = w.a.row[v0 + 1];
-    for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0));
+    const unsigned long long* bm0 = hub_bitmap(w.a, v0);
+    for (int pid = p.last[0]; pid != -1; pid = p.next[pid]) build(w, pid, w.a.col + l0, (int32_t)(r0 - l0), bm0);
     int64_t l1 = w.a.row[v1], r1 = w.a.row[v1 + 1];
+    const unsigned long long* bm1 = hub_bitmap(w.a, v1);
     for (int pid = p.last[1]; pid != -1; pid = p.next[pid]) {
-        build(w, pid, w.a.col + l1, (int32_t)(r1 - l1));
+        build(w, pid, w.a.col + l1, (int32_t)(r1 - l1), bm1);
         if (get_set(w, pid).len == 0 && pid < p.basic) {
             w.a.stats = stats;
             return;
@@ -1043,6 +1096,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     // A set that is never built reads as empty.
     for (int i = lane_id(); i < kMaxSets; i += kWave) {
         w.lds->sets[i].ptr = nullptr;
+        w.lds->sets[i].bm = nullptr;
         w.lds->sets[i].len = 0;
     }
     wave_sync();
@@ -1259,7 +1313,6 @@ struct BmArgs {
     int32_t k;                 // clique size
     int32_t max_d;             // roots with out-degree above this are left to the list engine
     unsigned long long* rows_scratch;  // per wave kBmMaxD * kBmMaxW words
-    int64_t nchunks;           // root chunks of kBmChunk vertices
     unsigned long long* out;   // the plan's sum cell
     Counters* ctr;
     int32_t stats;
@@ -1267,8 +1320,10 @@ struct BmArgs {
     int32_t big_cap_words;     // block-per-root kernel (its rows in LDS); 0: none do
     int32_t small_max_d;       // roots with k-1 <= d <= small_max_d (<= 64) go to the small-root kernel
     int32_t tri;               // 1: every row holds larger ids only, -1: smaller ids only, 0: neither
+    const int32_t* order;      // the vertices by out-degree, largest first (ties by id)
+    int32_t n_big;             // order[0, n_big): out-degree above 64
+    int32_t n_active;          // order[0, n_active): out-degree at least 2
 };
-constexpr int kBmChunk = 16;
 
 // Roots the block-per-root kernel takes (the per-wave kernel skips them).
 __device__ __forceinline__ bool bm_is_big(const BmArgs& a, int32_t d) {
@@ -1384,17 +1439,28 @@ __device__ __forceinline__ unsigned long long bm_count_pair_tri(const unsigned l
 #pragma unroll
         for (int x = 0; x < WMAX; ++x) {
             unsigned long long m = S[x];
+            // Two items per step, so that their row loads are in flight together.
             while (m) {
-                const int l = x * 64 + __builtin_ctzll(m);
+                const int l1 = x * 64 + __builtin_ctzll(m);
                 m &= m - 1;
-                const unsigned long long* rl = rows + (int64_t)l * W;
+                const unsigned long long* r1 = rows + (int64_t)l1 * W;
+                const bool two = m != 0ull;
+                const int l2 = two ? x * 64 + __builtin_ctzll(m) : l1;
+                if (two) m &= m - 1;
+                const unsigned long long* r2 = rows + (int64_t)l2 * W;
                 if constexpr (TRI > 0) {
 #pragma unroll
                     for (int y = x; y < WMAX; ++y)
-                        if (y < W) acc += __popcll(S[y] & rl[y]);
+                        if (y < W) {
+                            const unsigned long long a1 = r1[y], a2 = r2[y];
+                            acc += __popcll(S[y] & a1) + (two ? __popcll(S[y] & a2) : 0);
+                        }
                 } else {
 #pragma unroll
-                    for (int y = 0; y <= x; ++y) acc += __popcll(S[y] & rl[y]);
+                    for (int y = 0; y <= x; ++y) {
+                        const unsigned long long a1 = r1[y], a2 = r2[y];
+                        acc += __popcll(S[y] & a1) + (two ? __popcll(S[y] & a2) : 0);
+                    }
                 }
             }
         }
@@ -1489,14 +1555,14 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
     unsigned long long* gl_rows = a.rows_scratch + (int64_t)blockIdx.x * kBmMaxD * kBmMaxW;
     unsigned long long acc = 0, bytes = 0, wave_bytes = 0;
     unsigned long long roots = 0;
+    // Roots of out-degree above 64, heaviest first, one per claim.
     while (true) {
-        unsigned long long chunk = 0;
-        if (ln == 0) chunk = atomicAdd(&a.ctr->bm_cursor, 1ull);
-        chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
-        if ((long long)chunk >= a.nchunks) break;
-        const int32_t vbeg = (int32_t)(chunk * kBmChunk);
-        const int32_t vend = vbeg + kBmChunk < a.nv ? vbeg + kBmChunk : a.nv;
-        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
+        unsigned long long idx = 0;
+        if (ln == 0) idx = atomicAdd(&a.ctr->bm_cursor, 1ull);
+        idx = (unsigned long long)uni64((int64_t)__shfl((long long)idx, 0));
+        if ((long long)idx >= a.n_big) break;
+        {
+            const int32_t v0 = unii(a.order[idx]);
             int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
             int32_t d = unii((int32_t)(r0 - l0));
             if (d < K - 1 || d > a.max_d || d <= a.small_max_d || bm_is_big(a, d)) continue;
@@ -1530,7 +1596,6 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_kernel(BmArgs a) {
 // bm_root.
 // ---------------------------------------------------------------------------
 constexpr int kBigThreads = 512;
-constexpr int kBigChunk = 16;  // vertices per cursor claim (fine enough that the last claims balance)
 
 // Pair counting of the block kernel, sized to the root's row width.
 template <int WMAX, int K, int TRI>
@@ -1550,16 +1615,16 @@ __global__ void __launch_bounds__(kBigThreads) clique_bitmap_big_kernel(BmArgs a
     const int tid = threadIdx.x, ln = tid & (kWave - 1), wv = tid / kWave;
     constexpr int nwv = kBigThreads / kWave;
     unsigned long long acc = 0, bytes = 0, roots = 0;
-    const int64_t nchunks = ((int64_t)a.nv + kBigChunk - 1) / kBigChunk;
+    // Roots of out-degree above 64, heaviest first, one per claim: the
+    // last claims are the lightest roots, so the blocks finish together.
     while (true) {
         if (tid == 0) chunk_sh = (long long)atomicAdd(&a.ctr->big_cursor, 1ull);
         __syncthreads();
-        const long long chunk = chunk_sh;
+        const long long idx = chunk_sh;
         __syncthreads();
-        if (chunk >= nchunks) break;
-        const int32_t vbeg = (int32_t)(chunk * kBigChunk);
-        const int32_t vend = vbeg + kBigChunk < a.nv ? vbeg + kBigChunk : a.nv;
-        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
+        if (idx >= a.n_big) break;
+        {
+            const int32_t v0 = a.order[idx];
             const int64_t l0 = a.row[v0], r0 = a.row[v0 + 1];
             const int32_t d = (int32_t)(r0 - l0);
             if (d < K - 1 || !bm_is_big(a, d)) continue;
@@ -1748,15 +1813,19 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_small_kernel(BmArgs a) {
     __shared__ unsigned long long rows[kSmallD];
     const int ln = lane_id();
     unsigned long long acc = 0, bytes = 0, roots = 0;
-    const int64_t nchunks = ((int64_t)a.nv + kSmallChunk - 1) / kSmallChunk;
+    // Chunks of the roots of out-degree 2..64 in decreasing out-degree:
+    // each claim holds roots of one size, the heavy ones first.
+    const int64_t nsmall = (int64_t)a.n_active - a.n_big;
+    const int64_t nchunks = (nsmall + kSmallChunk - 1) / kSmallChunk;
     while (true) {
         unsigned long long chunk = 0;
         if (ln == 0) chunk = atomicAdd(&a.ctr->sm_cursor, 1ull);
         chunk = (unsigned long long)uni64((int64_t)__shfl((long long)chunk, 0));
         if ((int64_t)chunk >= nchunks) break;
-        const int32_t vbeg = (int32_t)(chunk * kSmallChunk);
-        const int32_t vend = vbeg + kSmallChunk < a.nv ? vbeg + kSmallChunk : a.nv;
-        for (int32_t v0 = vbeg; v0 < vend; ++v0) {
+        const int64_t ibeg = a.n_big + (int64_t)chunk * kSmallChunk;
+        const int64_t iend = ibeg + kSmallChunk < a.n_active ? ibeg + kSmallChunk : a.n_active;
+        for (int64_t oi = ibeg; oi < iend; ++oi) {
+            const int32_t v0 = unii(a.order[oi]);
             const int64_t l0 = a.row[v0];
             const int32_t d = unii((int32_t)(a.row[v0 + 1] - l0));
             if (d < K - 1 || d > a.small_max_d) continue;
@@ -1781,24 +1850,38 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_small_kernel(BmArgs a) {
             if (ln < d) off[ln] = incl - du;
             rows[ln] = 0ull;
             wave_sync();
-            for (int32_t t0 = 0; t0 < total; t0 += kWave) {
-                const int32_t t = t0 + ln;
-                if (t < total) {
-                    int32_t p = 0, n = d;  // the list holding element t: largest p with off[p] <= t
-                    while (n > 1) {
-                        const int32_t half = n >> 1;
-                        p = (off[p + half] <= t) ? p + half : p;
-                        n -= half;
+            // Four elements per lane per step, their loads in flight together.
+            constexpr int kIlp = 4;
+            for (int32_t t0 = 0; t0 < total; t0 += kIlp * kWave) {
+                uint32_t xs[kIlp];
+                int32_t ps[kIlp];
+#pragma unroll
+                for (int k = 0; k < kIlp; ++k) {
+                    const int32_t t = t0 + k * kWave + ln;
+                    ps[k] = -1;
+                    xs[k] = 0;
+                    if (t < total) {
+                        int32_t p = 0, n = d;  // the list holding element t: largest p with off[p] <= t
+                        while (n > 1) {
+                            const int32_t half = n >> 1;
+                            p = (off[p + half] <= t) ? p + half : p;
+                            n -= half;
+                        }
+                        ps[k] = p;
+                        xs[k] = a.col[beg[p] + (t - off[p])];
                     }
-                    const uint32_t x = a.col[beg[p] + (t - off[p])];
-                    int32_t q = 0;  // position of x in N(v0)
-                    n = d;
+                }
+#pragma unroll
+                for (int k = 0; k < kIlp; ++k) {
+                    if (ps[k] < 0) continue;
+                    const uint32_t x = xs[k];
+                    int32_t q = 0, n = d;  // position of x in N(v0)
                     while (n > 1) {
                         const int32_t half = n >> 1;
                         q = (nbr[q + half] <= x) ? q + half : q;
                         n -= half;
                     }
-                    if (nbr[q] == x) atomicOr(&rows[p], 1ull << q);
+                    if (nbr[q] == x) atomicOr(&rows[ps[k]], 1ull << q);
                 }
             }
             wave_sync();

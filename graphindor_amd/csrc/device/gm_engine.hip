@@ -44,6 +44,21 @@ __global__ void accumulate_counters_kernel(Counters* total, const Counters* part
     total->timed_out |= part->timed_out;
 }
 
+// Hub bitmaps: block b writes the bitmap of hub hubs[b] (its neighbour set
+// over vertex ids) and records its slot.
+__global__ void hub_build_kernel(const int64_t* row, const uint32_t* col, const int32_t* hubs, int64_t words,
+                                 unsigned long long* bits, int32_t* slot) {
+    const int32_t h = hubs[blockIdx.x];
+    unsigned long long* bm = bits + (int64_t)blockIdx.x * words;
+    for (int64_t i = threadIdx.x; i < words; i += blockDim.x) bm[i] = 0ull;
+    __syncthreads();
+    for (int64_t e = row[h] + threadIdx.x; e < row[h + 1]; e += blockDim.x) {
+        const uint32_t x = col[e];
+        atomicOr(&bm[x >> 6], 1ull << (x & 63u));
+    }
+    if (threadIdx.x == 0) slot[h] = (int32_t)blockIdx.x;
+}
+
 // Zeroes n 64-bit words (a kernel, so that it may be captured in a graph).
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -148,6 +163,14 @@ int32_t slice_len_of(const gm_tuning* t) {
     return (t && t->slice_len > 0) ? t->slice_len : kDefaultSliceLen;
 }
 
+// Father elements the pattern tail stages in LDS: gm_tuning.lds_set_cap
+// (0 = the whole per-wave arena, -1 = never).
+int32_t stage_cap_of(const gm_tuning* t) {
+    if (t && t->lds_set_cap < 0) return -1;
+    if (t && t->lds_set_cap > 0) return std::min<int32_t>(t->lds_set_cap, kStage);
+    return kStage;
+}
+
 hipError_t alloc_queue(SliceItem** q) {
     if (*q) return hipSuccess;
     hipError_t e = hipMalloc((void**)q, sizeof(SliceItem) * kQueueCap);
@@ -191,8 +214,8 @@ hipError_t big_rows_capacity(int device, int32_t* cap_words, size_t* dyn_bytes) 
     int maxsh = 0;
     hipError_t e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) return e;
-    const size_t static_bytes = sizeof(uint32_t) * kBmMaxD + sizeof(int32_t) * (kBmMaxD + 1) + 64 +
-                                sizeof(unsigned long long) * 2 * (kBigThreads / kWave) + 64;
+    const size_t static_bytes = sizeof(uint32_t) * kBmMaxD + sizeof(int32_t) * (kBmMaxD + 1) +
+                                64 + sizeof(unsigned long long) * 2 * (kBigThreads / kWave) + 64;
     const size_t avail = (size_t)maxsh > static_bytes ? (size_t)maxsh - static_bytes : 0;
     const size_t words = std::min<size_t>(avail / sizeof(unsigned long long), (size_t)kBmMaxD * kBmMaxW);
     *cap_words = (int32_t)words;
@@ -317,7 +340,63 @@ struct gm_graph {
     int32_t max_deg;
     int32_t second_deg;
     int32_t tri;  // 1: every row holds larger ids only; -1: smaller ids only; 0: neither (or no edges)
+    int32_t* d_order;  // vertices by out-degree, largest first (ties by id)
+    int32_t n_big;     // out-degree above 64 (the bitmap engine's block and per-wave roots)
+    int32_t n_active;  // out-degree at least 2
+    std::vector<int32_t> h_order;  // d_order on the host
+    std::vector<int32_t> h_deg;    // out-degree of h_order[i]
+    // Hub bitmaps (pattern rule), built at first use for a threshold.
+    int32_t hub_deg_built = 0;     // threshold they were built for (0: none built)
+    int64_t n_hubs = 0;
+    int64_t hub_words = 0;
+    unsigned long long* d_hub_bits = nullptr;
+    int32_t* d_hub_slot = nullptr;
 };
+
+namespace {
+
+constexpr int32_t kDefaultHubDegree = 4096;
+constexpr size_t kHubBytesCap = (size_t)4 << 30;
+
+// Builds (or reuses) the hub bitmaps of graph g for threshold T on stream s.
+int ensure_hub_bitmaps(gm_graph* g, int32_t T, hipStream_t s) {
+    if (g->hub_deg_built == T) return GM_OK;
+    if (g->d_hub_bits) (void)hipFree(g->d_hub_bits);
+    g->d_hub_bits = nullptr;
+    g->n_hubs = 0;
+    g->hub_deg_built = 0;
+    g->hub_words = ((int64_t)g->n + 63) / 64;
+    int64_t H = 0;
+    while (H < (int64_t)g->h_deg.size() && g->h_deg[(size_t)H] >= T) ++H;
+    const size_t per = (size_t)g->hub_words * sizeof(unsigned long long);
+    if (per > 0 && (size_t)H * per > kHubBytesCap) H = (int64_t)(kHubBytesCap / per);
+    if (H > 0) {
+        if (!g->d_hub_slot) HIP_TRY(hipMalloc(&g->d_hub_slot, sizeof(int32_t) * (size_t)g->n));
+        HIP_TRY(hipMalloc(&g->d_hub_bits, per * (size_t)H));
+        int32_t* d_hubs = nullptr;
+        HIP_TRY(hipMalloc(&d_hubs, sizeof(int32_t) * (size_t)H));
+        hipError_t e = hipMemsetAsync(g->d_hub_slot, 0xff, sizeof(int32_t) * (size_t)g->n, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_hubs, g->h_order.data(), sizeof(int32_t) * (size_t)H, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(hub_build_kernel, dim3((unsigned)H), dim3(256), 0, s, g->d_row, g->d_col, d_hubs,
+                               g->hub_words, g->d_hub_bits, g->d_hub_slot);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d_hubs);
+        if (e != hipSuccess) {
+            (void)hipFree(g->d_hub_bits);
+            g->d_hub_bits = nullptr;
+            return fail(GM_ERR_HIP, std::string("hub bitmaps: ") + hipGetErrorString(e));
+        }
+    }
+    g->n_hubs = H;
+    g->hub_deg_built = T;
+    return GM_OK;
+}
+
+}  // namespace
 
 struct gm_batch;
 
@@ -375,24 +454,49 @@ int gm_graph_create(const int64_t* row_ptr, const int32_t* col, int32_t n, int64
     g->max_deg = (int32_t)d1;
     g->second_deg = (int32_t)d2;
     g->tri = upper ? 1 : (lower ? -1 : 0);
+    // The bitmap kernels claim their roots heaviest first: a counting sort of
+    // the vertices by decreasing out-degree, stable in id.
+    std::vector<int32_t> order((size_t)n);
+    {
+        std::vector<int64_t> start((size_t)d1 + 2, 0);
+        for (int32_t v = 0; v < n; ++v) ++start[(size_t)(d1 - (row_ptr[v + 1] - row_ptr[v])) + 1];
+        for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+        for (int32_t v = 0; v < n; ++v) order[(size_t)start[(size_t)(d1 - (row_ptr[v + 1] - row_ptr[v]))]++] = v;
+    }
+    g->h_deg.resize((size_t)n);
+    for (int32_t i = 0; i < n; ++i) g->h_deg[(size_t)i] = (int32_t)(row_ptr[order[(size_t)i] + 1] - row_ptr[order[(size_t)i]]);
+    g->n_big = 0;
+    g->n_active = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        const int64_t d = row_ptr[v + 1] - row_ptr[v];
+        g->n_big += d > kSmallD ? 1 : 0;
+        g->n_active += d >= 2 ? 1 : 0;
+    }
+    g->d_order = nullptr;
     g->d_row = nullptr;
     g->d_col = nullptr;
     hipError_t e1 = hipMalloc(&g->d_row, sizeof(int64_t) * ((size_t)n + 1));
     hipError_t e2 = hipMalloc(&g->d_col, sizeof(uint32_t) * (size_t)(nnz > 0 ? nnz : 1));
-    if (e1 != hipSuccess || e2 != hipSuccess) {
+    hipError_t e3 = hipMalloc(&g->d_order, sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
         if (g->d_row) (void)hipFree(g->d_row);
         if (g->d_col) (void)hipFree(g->d_col);
+        if (g->d_order) (void)hipFree(g->d_order);
         delete g;
         return fail(GM_ERR_OOM, "gm_graph_create: device allocation failed");
     }
     hipError_t c1 = hipMemcpy(g->d_row, row_ptr, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyHostToDevice);
     hipError_t c2 = nnz > 0 ? hipMemcpy(g->d_col, col, sizeof(uint32_t) * (size_t)nnz, hipMemcpyHostToDevice) : hipSuccess;
-    if (c1 != hipSuccess || c2 != hipSuccess) {
+    hipError_t c3 = n > 0 ? hipMemcpy(g->d_order, order.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice)
+                          : hipSuccess;
+    if (c1 != hipSuccess || c2 != hipSuccess || c3 != hipSuccess) {
         (void)hipFree(g->d_row);
         (void)hipFree(g->d_col);
+        (void)hipFree(g->d_order);
         delete g;
         return fail(GM_ERR_HIP, "gm_graph_create: upload failed");
     }
+    g->h_order = std::move(order);
     *out = g;
     return GM_OK;
 }
@@ -401,6 +505,9 @@ void gm_graph_destroy(gm_graph* g) {
     if (!g) return;
     (void)hipFree(g->d_row);
     (void)hipFree(g->d_col);
+    (void)hipFree(g->d_order);
+    if (g->d_hub_bits) (void)hipFree(g->d_hub_bits);
+    if (g->d_hub_slot) (void)hipFree(g->d_hub_slot);
     delete g;
 }
 
@@ -751,6 +858,21 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
     a.task_max = task_max;
+    a.stage_cap = stage_cap_of(t);
+    // Hub bitmaps for the pattern rule (not in the static workspace, which
+    // allocates nothing).
+    int64_t hubs_used = 0;
+    const int32_t hub_deg = (t && t->hub_degree != 0) ? t->hub_degree : kDefaultHubDegree;
+    if (np_pat > 0 && !fixed_ws && hub_deg > 0 && g->max_deg >= hub_deg) {
+        const int rc = ensure_hub_bitmaps(const_cast<gm_graph*>(g), hub_deg, s);
+        if (rc != GM_OK) return rc;
+        if (g->n_hubs > 0) {
+            a.hub_bits = g->d_hub_bits;
+            a.hub_slot = g->d_hub_slot;
+            a.hub_words = g->hub_words;
+            hubs_used = g->n_hubs;
+        }
+    }
     // Slice donation applies to the pattern rule; clique plans rely on the
     // degree orientation and the bitmap engine to bound a root's work.
     if (np_pat > 0 && !fixed_ws) {
@@ -797,6 +919,8 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
     // clique root's work).
     a.queue = nullptr;
     a.queue_cap = 0;
+    a.hub_bits = nullptr;
+    a.hub_slot = nullptr;
     for (int c = 0; c < np_clq; ++c) {
         const int evl = 2 + 6 * c, evb = evl + 2, evk = evl + 4;
         const gm_plan* p = plans[order[np_pat + c]];
@@ -814,7 +938,9 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
             b.k = p->host.n;
             b.max_d = bm_max;
             b.rows_scratch = nullptr;
-            b.nchunks = ((int64_t)g->n + kBmChunk - 1) / kBmChunk;
+            b.order = g->d_order;
+            b.n_big = g->n_big;
+            b.n_active = g->n_active;
             b.out = cell;
             b.ctr = ctr_bm;
             b.stats = stats_on;
@@ -952,6 +1078,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
         stats->bitmap_alg_bytes = cb.bytes;
         stats->bitmap_block_ms = block_ms;
         stats->bitmap_block_alg_bytes = ck.bytes;
+        stats->hub_vertices = hubs_used;
         fill_queue_stats(c, stats);
     }
     return GM_OK;
@@ -1074,6 +1201,23 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     a.tail_off = (t && t->lane_tail < 0) ? 1 : 0;
     a.tail_work = (t && t->lane_tail > 0) ? t->lane_tail : kLaneTailWork;
     a.task_max = (t && t->chunk_edges > 0) ? t->chunk_edges : kWave;
+    a.stage_cap = stage_cap_of(t);
+    {
+        // The hub bitmaps are built before the capture and belong to the graph.
+        const int32_t hub_deg = (t && t->hub_degree != 0) ? t->hub_degree : kDefaultHubDegree;
+        if (hub_deg > 0 && g->max_deg >= hub_deg) {
+            const int rc = ensure_hub_bitmaps(const_cast<gm_graph*>(g), hub_deg, nullptr);
+            if (rc != GM_OK) {
+                gm_batch_destroy(b);
+                return rc;
+            }
+            if (g->n_hubs > 0) {
+                a.hub_bits = g->d_hub_bits;
+                a.hub_slot = g->d_hub_slot;
+                a.hub_words = g->hub_words;
+            }
+        }
+    }
     if (slice_len_of(t) > 0) {
         if (alloc_queue(&b->d_queue) != hipSuccess) return bail(GM_ERR_OOM, "gm_batch_create: queue allocation failed");
         a.queue = b->d_queue;
@@ -1153,6 +1297,7 @@ int gm_batch_run(gm_batch* b, void* stream, uint64_t* raw_sums, gm_stats* stats)
         stats->bitmap_alg_bytes = 0;
         stats->bitmap_block_ms = 0;
         stats->bitmap_block_alg_bytes = 0;
+        stats->hub_vertices = 0;
         fill_queue_stats(c, stats);
     }
     return GM_OK;

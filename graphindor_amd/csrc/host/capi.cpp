@@ -429,13 +429,15 @@ GS_API int gs_costmodel_predict(void* h, const float* x, int32_t rows, float* y)
 // graph `gp` (sample_roots roots, time_limit_ms each); the fastest is kept.
 // info = {candidates, chosen, analytic, learned}, *select_ms = time spent.
 GS_API void* gs_schedule_select_measured(int n, const char* adj, int iep, void* gp, void* model, int top_k,
-                                         int sample_roots, int time_limit_ms, int32_t* info, double* select_ms) {
+                                         int sample_roots, int64_t root_stride, int time_limit_ms, int32_t* info,
+                                         double* select_ms) {
     void* out = nullptr;
     guarded([&] {
         Pattern p(n, adj);
         MeasureOptions opt;
         opt.top_k = top_k;
         opt.sample_roots = sample_roots;
+        opt.root_stride = root_stride;
         opt.time_limit_ms = time_limit_ms;
         ScheduleChoice ch;
         Schedule_IEP* s = select_schedule_measured(p, *(Graph*)gp, (CostModel*)model, iep != 0, opt, &ch);

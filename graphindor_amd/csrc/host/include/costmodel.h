@@ -108,7 +108,8 @@ struct ScheduleChoice {
 // (S = nnz / sample_roots) with a time limit each, and the fastest is kept.
 struct MeasureOptions {
     int top_k = 3;
-    int sample_roots = 4096;
+    int sample_roots = 4096;  // roots of the timing sample: edges e % (nnz / sample_roots) == 0
+    int64_t root_stride = 0;  // > 0: time on the sample e % root_stride == 0 instead
     int time_limit_ms = 200;
 };
 

@@ -56,7 +56,8 @@ class GmTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("lds_set_cap", ctypes.c_int32), ("bitmap_max_degree", ctypes.c_int32),
                 ("lane_tail", ctypes.c_int32), ("root_stride", ctypes.c_int32), ("slice_len", ctypes.c_int32),
-                ("bitmap_block_degree", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32)]
+                ("bitmap_block_degree", ctypes.c_int32), ("time_limit_ms", ctypes.c_int32),
+                ("hub_degree", ctypes.c_int32)]
 
 
 class GmStats(ctypes.Structure):
@@ -65,7 +66,8 @@ class GmStats(ctypes.Structure):
                 ("bitmap_ms", ctypes.c_double), ("bitmap_alg_bytes", ctypes.c_uint64),
                 ("bitmap_block_ms", ctypes.c_double), ("bitmap_block_alg_bytes", ctypes.c_uint64),
                 ("slices", ctypes.c_uint64), ("donations", ctypes.c_uint64), ("queue_retries", ctypes.c_uint64),
-                ("slice_wave_ms", ctypes.c_double), ("idle_wave_ms", ctypes.c_double)]
+                ("slice_wave_ms", ctypes.c_double), ("idle_wave_ms", ctypes.c_double),
+                ("hub_vertices", ctypes.c_int64)]
 
 
 _gm = None
@@ -192,7 +194,8 @@ def gs():
             ("gs_costmodel_predict", [VP, VP, ctypes.c_int32, VP], ctypes.c_int),
             ("gs_costmodel_set_margin", [VP, ctypes.c_float], None),
             ("gs_schedule_select_measured", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, VP, ctypes.c_int,
-                                             ctypes.c_int, ctypes.c_int, VP, ctypes.POINTER(ctypes.c_double)], VP),
+                                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, VP,
+                                             ctypes.POINTER(ctypes.c_double)], VP),
             ("gs_costmodel_margin", [VP], ctypes.c_float),
             ("gs_schedule_select", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64, VP,
                                     VP, ctypes.POINTER(ctypes.c_double)], VP),
@@ -488,14 +491,16 @@ def select_schedule(n, adj, vertex, tri_cnt, model=None, iep=True):
                "select_ms": ms.value}
 
 
-def select_schedule_measured(n, adj, graph, model=None, iep=True, top_k=3, sample_roots=4096, time_limit_ms=200):
+def select_schedule_measured(n, adj, graph, model=None, iep=True, top_k=3, sample_roots=4096, time_limit_ms=200,
+                             root_stride=0):
     """(Schedule, info): the model's top_k proposals and the analytic pick
     timed on a root sample of `graph` (a host Graph; its device copy is
-    used), the fastest kept."""
+    used) -- about sample_roots roots, or exactly the edges e % root_stride
+    == 0 when root_stride > 0 -- the fastest kept."""
     info = (ctypes.c_int32 * 4)()
     ms = ctypes.c_double(0)
     h = gs().gs_schedule_select_measured(n, adj.encode(), 1 if iep else 0, graph.h, model.h if model is not None else None,
-                                         top_k, sample_roots, time_limit_ms, info, ctypes.byref(ms))
+                                         top_k, sample_roots, root_stride, time_limit_ms, info, ctypes.byref(ms))
     s = Schedule(h)
     return s, {"candidates": info[0], "chosen": info[1], "analytic": info[2], "learned": bool(info[3]),
                "select_ms": ms.value}
@@ -595,8 +600,10 @@ def count_static(dgraph, dplan, flags=0, **tuning):
 
 
 def _tuning(waves_per_cu=0, bitmap_max_degree=0, lane_tail=0, chunk_edges=0, root_stride=0, slice_len=0,
-            bitmap_block_degree=0, time_limit_ms=0):
+            bitmap_block_degree=0, time_limit_ms=0, hub_degree=0, lds_set_cap=0):
     tun = GmTuning()
+    tun.hub_degree = hub_degree
+    tun.lds_set_cap = lds_set_cap
     tun.time_limit_ms = time_limit_ms
     tun.bitmap_block_degree = bitmap_block_degree
     tun.root_stride = root_stride

@@ -98,7 +98,9 @@ typedef struct gm_tuning {
     int32_t waves_per_cu;      /* resident waves per CU of the persistent grid */
     int32_t chunk_edges;       /* most root edges one work-steal claims (claims shrink towards 1 as the
                                   work runs out; 0 = 64) */
-    int32_t lds_set_cap;       /* max elements of a prefix set staged in LDS */
+    int32_t lds_set_cap;       /* GM_MODE_PATTERN: the last candidate level stages its father sets in
+                                  LDS when they hold at most this many elements in all
+                                  (0 = default 2048, the per-wave arena; -1 = never) */
     int32_t bitmap_max_degree; /* GM_MODE_CLIQUE: roots of out-degree up to this use the
                                   local-bitmap engine (0 = default 1024, -1 = off) */
     int32_t lane_tail;         /* GM_MODE_PATTERN: last candidate depth counted one candidate per lane
@@ -117,6 +119,10 @@ typedef struct gm_tuning {
     int32_t time_limit_ms;     /* GM_MODE_PATTERN / list engine: waves stop once they have run this long
                                   and the call fails with GM_ERR_TIMEOUT (0 = no limit; for timing
                                   harnesses that must not wait on a slow candidate) */
+    int32_t hub_degree;        /* GM_MODE_PATTERN: vertices of degree at least this carry their
+                                  neighbour set as a bitmap over vertex ids, so that a lookup in a hub's
+                                  list is one load (0 = default 4096, -1 = none; the bitmaps, n/8 bytes
+                                  per hub, at most 4 GiB, are built on the graph at first use) */
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */
@@ -137,6 +143,7 @@ typedef struct gm_stats {
     uint64_t queue_retries;   /* queue-head claims lost to another wave */
     double slice_wave_ms;     /* wall time waves spent on donated slices, summed over waves */
     double idle_wave_ms;      /* wall time waves spent waiting for slices, summed over waves */
+    int64_t hub_vertices;     /* hub bitmaps the pattern rule used (gm_tuning.hub_degree) */
 } gm_stats;
 
 /* gm_count flags */

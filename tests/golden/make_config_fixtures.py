@@ -38,11 +38,11 @@ MOTIFS4 = ["0111101111011110", "0111101111001100", "0111101011001000", "01111000
            "0110100110000100"]
 PM5 = {"house": "0101110101010101010011000", "c5": "0100110100010100010110010",
        "k4_pendant": "0111110110110101110010000"}
-PM6 = {"c6": "010001101000010100001010000101100010", "prism": "011100101010110001100011010101001110"}
+PM6 = {"prism": "011100101010110001100011010101001110", "c6": "010001101000010100001010000101100010"}
 # Root samples: the strides keep each sampled count within minutes of oracle
 # time here and seconds of device time.
 STRIDES22 = {"house": 4096, "c5": 4194304, "k4_pendant": 256}
-STRIDES23 = {"c6": 4194304, "prism": 1048576}
+STRIDES23 = {"prism": 1048576, "c6": 33554432}
 
 
 def clique_adj(k):
@@ -90,7 +90,7 @@ def section_batch(threads, stride=256):
     return {"batch512_s5": rec}
 
 
-def section_pm(threads, key, make, n, patterns, strides):
+def section_pm(threads, key, make, n, patterns, strides, save):
     g = make()
     inf = g.info()
     v, e = (x.copy() for x in g.csr())
@@ -98,6 +98,7 @@ def section_pm(threads, key, make, n, patterns, strides):
     rec = {"vertices": int(inf["v_cnt"]), "csr_entries": int(inf["e_cnt"]), "tri_cnt": int(inf["tri_cnt"]),
            "patterns": {}}
     for name, adj in patterns.items():
+        save({key: rec})  # each pattern is kept as soon as it is counted
         s, _ = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
         S = strides[name]
         t = time.time()
@@ -105,6 +106,7 @@ def section_pm(threads, key, make, n, patterns, strides):
         rec["patterns"][name] = {"adj": adj, "stride": S, "raw": int(raw), "alg_bytes": int(b),
                                  "schedule": schedule_digest(s)}
         print(f"{key} {name} stride {S}: raw {raw} bytes {b} ({time.time() - t:.1f} s)", flush=True)
+    save({key: rec})
     return {key: rec}
 
 
@@ -114,20 +116,27 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     a = ap.parse_args()
     data = json.load(open(OUT)) if os.path.exists(OUT) else {}
+
+    def save(part):
+        data.update(part)
+        with open(OUT + ".tmp", "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+            f.write("\n")
+        os.replace(OUT + ".tmp", OUT)
+
     for sec in a.sections:
         if sec == "rmat20":
             data.update(section_rmat20(a.threads))
         elif sec == "batch":
             data.update(section_batch(a.threads))
         elif sec == "rmat22":
-            data.update(section_pm(a.threads, "rmat22_s4", lambda: N.Graph.rmat(22, 16, 4), 5, PM5, STRIDES22))
+            data.update(section_pm(a.threads, "rmat22_s4", lambda: N.Graph.rmat(22, 16, 4), 5, PM5, STRIDES22, save))
         elif sec == "rmat23":
-            data.update(section_pm(a.threads, "rmat23_ef32_s7", lambda: N.Graph.rmat(23, 32, 7), 6, PM6, STRIDES23))
+            data.update(section_pm(a.threads, "rmat23_ef32_s7", lambda: N.Graph.rmat(23, 32, 7), 6, PM6, STRIDES23,
+                                   save))
         else:
             raise SystemExit(f"unknown section {sec}")
-        with open(OUT, "w") as f:
-            json.dump(data, f, indent=1, sort_keys=True)
-            f.write("\n")
+        save({})
     print("wrote", OUT)
 
 

@@ -73,8 +73,8 @@ def rmat22():
 
 @pytest.mark.parametrize("name", ["house", "c5", "k4_pendant"])
 def test_config_pm_rmat22_root_sample(fixtures, rmat22, name):
-    if "rmat22_s4" not in fixtures:
-        pytest.skip("rmat22_s4 not in the fixtures")
+    if name not in fixtures.get("rmat22_s4", {}).get("patterns", {}):
+        pytest.skip(f"rmat22_s4 {name} not in the fixtures")
     inf, v, e, dg = rmat22
     want = fixtures["rmat22_s4"]["patterns"][name]
     assert len(e) == fixtures["rmat22_s4"]["csr_entries"]
@@ -109,18 +109,24 @@ def test_config_batch512_motifs(fixtures):
     b.free()
 
 
-@pytest.mark.parametrize("name", ["c6", "prism"])
-def test_config_learned_rmat23_root_sample(fixtures, name):
-    if "rmat23_ef32_s7" not in fixtures:
+@pytest.fixture(scope="module")
+def rmat23(fixtures):
+    if not fixtures.get("rmat23_ef32_s7", {}).get("patterns"):
         pytest.skip("rmat23_ef32_s7 not in the fixtures")
-    want = fixtures["rmat23_ef32_s7"]["patterns"][name]
     g = N.Graph.rmat(23, 32, 7)
-    inf = g.info()
     v, e = (x.copy() for x in g.csr())
+    return g.info(), v, e, N.DeviceGraph(v, e)
+
+
+@pytest.mark.parametrize("name", ["c6", "prism"])
+def test_config_learned_rmat23_root_sample(fixtures, rmat23, name):
+    if name not in fixtures["rmat23_ef32_s7"]["patterns"]:
+        pytest.skip(f"rmat23_ef32_s7 {name} not in the fixtures")
+    want = fixtures["rmat23_ef32_s7"]["patterns"][name]
+    inf, v, e, dg = rmat23
     assert len(e) == fixtures["rmat23_ef32_s7"]["csr_entries"]
     s, _ = N.select_schedule(6, want["adj"], v, inf["tri_cnt"], None)
     assert digest(s) == want["schedule"]
-    dg = N.DeviceGraph(v, e)
     print(f"case: rmat23 {name} stride {want['stride']}", flush=True)
     raw, st = N.count_batch(dg, [0, len(e)], [N.DevicePlan(s.plan_desc())], flags=N.GM_COUNT_ALG_BYTES,
                             root_stride=want["stride"])

@@ -561,3 +561,90 @@ def test_time_limit_fails_cleanly_and_leaves_no_state():
     assert N.count(sdg, sdp)[0] == want
     assert N.count(sdg, sdp, flags=N.GM_COUNT_EAGER_DONATION, slice_len=4)[0] == want
     assert N.count(sdg, sdp, time_limit_ms=60000)[0] == want
+
+
+def test_clique_time_limit_fails_cleanly():
+    """The clique rule's list engine also honours the time limit: a count
+    that passes it fails with GM_ERR_TIMEOUT, alone and beside a pattern plan
+    (whose slice queue must then be left clear), and the next counts are exact."""
+    g = N.Graph.rmat(14, 16, 4)
+    v, e = (x.copy() for x in g.csr())
+    inf = g.info()
+    rv, re_ = reduce_for_clique(v, e)
+    k6 = N.Schedule.plain(6, clique_adj(6))
+    k6.redundancy = 1
+    cplan = N.DevicePlan(k6.plan_desc(N.GM_MODE_CLIQUE))
+    rdg = N.DeviceGraph(rv, re_)
+    with pytest.raises(N.GmTimeout):
+        N.count(rdg, cplan, time_limit_ms=1, bitmap_max_degree=-1)
+    # a pattern plan and a clique plan in one call, on the symmetric graph
+    dg = N.DeviceGraph(v, e)
+    c5 = N.Schedule.build(5, PATTERNS["c5"][1], inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"])
+    with pytest.raises(N.GmTimeout):
+        N.count_multi(dg, [N.DevicePlan(c5.plan_desc()), N.DevicePlan(k6.plan_desc(N.GM_MODE_CLIQUE))],
+                      flags=N.GM_COUNT_EAGER_DONATION, time_limit_ms=2, slice_len=4, bitmap_max_degree=-1)
+    small = N.Graph.rmat(10, 8, 17)
+    sv, se = (x.copy() for x in small.csr())
+    srv, sre = reduce_for_clique(sv, se)
+    sdesc = k6.plan_desc(N.GM_MODE_CLIQUE)
+    assert N.count(N.DeviceGraph(srv, sre), N.DevicePlan(sdesc), bitmap_max_degree=-1)[0] == \
+        O.count_clique(O.CSR(srv, sre), sdesc)[0]
+    sinf = small.info()
+    ss = N.Schedule.build(5, PATTERNS["c5"][1], sinf["v_cnt"], sinf["e_cnt"], sinf["tri_cnt"])
+    assert N.count(N.DeviceGraph(sv, se), N.DevicePlan(ss.plan_desc()), flags=N.GM_COUNT_EAGER_DONATION,
+                   slice_len=4)[0] == O.count_edge_rooted(O.CSR(sv, se), ss.plan_desc())[0]
+
+
+@pytest.mark.parametrize("hub", [8, 64, -1])
+@pytest.mark.parametrize("name", ["house", "c5", "k4_pendant", "c6", "prism", "tailed_triangle", "c4", "diamond"])
+def test_hub_bitmaps_keep_counts_and_bytes(name, hub):
+    """Lookups in hub neighbour lists by bitmap (every vertex of degree at
+    least `hub`; -1: none) give the oracle's sum and byte tally, and with the
+    lane-parallel last level off (whose byte model the oracle does not
+    follow) the sum and the tally of the same run without bitmaps.  R-MAT
+    2^10 ef4 keeps the oracle to seconds for every pattern."""
+    n, adj = PATTERNS[name]
+    g = N.Graph.rmat(10, 4, 8)
+    v, e = (x.copy() for x in g.csr())
+    s = N.Schedule.build(n, adj, len(v) - 1, len(e), g.info()["tri_cnt"])
+    desc = s.plan_desc()
+    dg, dp = N.DeviceGraph(v, e), N.DevicePlan(desc)
+    want = O.count_edge_rooted(O.CSR(v, e), desc)
+    print(f"case: {name} hub={hub}", flush=True)
+    raw, st = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, hub_degree=hub)
+    assert (raw, st.alg_bytes) == want
+    assert (st.hub_vertices > 0) == (hub > 0)
+    off, st_off = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, hub_degree=hub, lane_tail=-1)
+    plain, st_plain = N.count(dg, dp, flags=N.GM_COUNT_ALG_BYTES, hub_degree=-1, lane_tail=-1)
+    assert off == plain == want[0]
+    assert st_off.alg_bytes == st_plain.alg_bytes
+
+
+@pytest.mark.parametrize("cap", [-1, 16, 0])
+def test_tail_staging_cap_keeps_counts(wiki, cap):
+    """The last level's fathers staged in LDS up to lds_set_cap elements (-1:
+    never) give the same sums and bytes."""
+    v, e = wiki
+    for name in ("house", "k4_pendant"):
+        n, adj = PATTERNS[name]
+        s = N.Schedule.build(n, adj, len(v) - 1, len(e), 608389)
+        desc = s.plan_desc()
+        raw, st = N.count(N.DeviceGraph(v, e), N.DevicePlan(desc), flags=N.GM_COUNT_ALG_BYTES, lds_set_cap=cap)
+        assert (raw, st.alg_bytes) == O.count_edge_rooted(O.CSR(v, e), desc)
+
+
+def test_hub_bitmaps_in_captured_batch():
+    """A captured batch whose graph has hub bitmaps replays the per-graph sums."""
+    g, voff, eoff = N.Graph.batch(12, 9, 8.0, 11.0, 4.0, 24.0)
+    inf = g.info()
+    v, e = (x.copy() for x in g.csr())
+    dg = N.DeviceGraph(v, e)
+    plans = [N.DevicePlan(N.Schedule.build(4, adj, inf["v_cnt"], inf["e_cnt"], inf["tri_cnt"]).plan_desc())
+             for adj in MOTIFS4]
+    want, _ = N.count_batch(dg, eoff, plans, hub_degree=-1)
+    got, st = N.count_batch(dg, eoff, plans, hub_degree=16)
+    assert got == want and st.hub_vertices > 0
+    b = N.Batch(dg, eoff, plans, hub_degree=16)
+    for _ in range(3):
+        assert b.run()[0] == want
+    b.free()
