@@ -35,7 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # kernel), from two rocprofv3 PMC runs of this bench (FETCH_SIZE and
 # WRITE_SIZE, each scaled by what it reads for a 1 GiB device copy in the
 # same run).  None where no measurement of the current kernel exists.
-TRAFFIC = {}
+TRAFFIC = {
+    # FETCH_SIZE 3,554,610 KiB x 2.000 + WRITE_SIZE 1,939 KiB x 1.000 per
+    # dispatch (mean of 4); the 1 GiB copy read 524,306 KiB and wrote
+    # 1,048,576 KiB.
+    (20, "rank", "clique_bitmap_big_kernel<5>"): 7281826304,
+}
 
 
 def log(*a):
@@ -96,7 +101,7 @@ def host_cpu():
             "visible_cpus": visible}, physical
 
 
-def cpu_baseline(N, O, graphs, k, target_s=20.0):
+def cpu_baseline(N, O, graphs, k, host, target_s=20.0):
     """The oracle's restatement of the reference CPU engine (vertex-rooted,
     OpenMP dynamic; src/graph.cpp:305-566) counting k-cliques, on every
     physical core with OMP_PROC_BIND=close.  graphs: [(label, vertex, edge)]; the first is the
@@ -104,8 +109,11 @@ def cpu_baseline(N, O, graphs, k, target_s=20.0):
     id orientation.  Each is counted in full when a first sample says that
     fits the budget; otherwise on an even sample of the work (every root,
     depth-1 candidates i = 1 mod S), reported as a partial-progress rate
-    with the extrapolated full time labelled as such."""
-    cpu, threads = host_cpu()
+    with the extrapolated full time labelled as such.  host: host_cpu()'s
+    result, taken before any OpenMP runtime loaded (with OMP_PROC_BIND set,
+    libgomp binds the initial thread to one CPU when it starts, after which
+    the process's affinity mask shows one CPU)."""
+    cpu, threads = host
     out = {"unit": "s", "cores": threads, "kind": "port", "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
            "cpu": cpu, "graphs": {}}
     for label, v, e in graphs:
@@ -119,9 +127,17 @@ def cpu_baseline(N, O, graphs, k, target_s=20.0):
             sched = N.Schedule.plain(k, clique_adj(k))
             sched.redundancy = 1
         desc = sched.plan_desc()
-        t0 = time.perf_counter()
-        O.count_vertex_rooted_strided(csr, desc, threads, 64, 0)
-        est = (time.perf_counter() - t0) * 64
+        # Probe on ever larger samples (1/4096 of the depth-1 candidates,
+        # then 1/512, 1/64) until one takes a second, to estimate the full
+        # count's time without running past the budget.
+        for probe in (4096, 512, 64):
+            t0 = time.perf_counter()
+            O.count_vertex_rooted_strided(csr, desc, threads, probe, 0)
+            dt = time.perf_counter() - t0
+            est = dt * probe
+            log(f"cpu baseline {label}: 1/{probe} sample in {dt:.2f} s (full count ~{est:.0f} s)")
+            if dt >= 1.0:
+                break
         if est <= target_s * 1.5:
             t0 = time.perf_counter()
             raw = O.count_vertex_rooted(csr, desc, threads)
@@ -135,6 +151,7 @@ def cpu_baseline(N, O, graphs, k, target_s=20.0):
             rec = {"time_s": round(t * stride, 3), "measured": f"partial: 1/{stride} of the depth-1 candidates of every "
                    f"root in {t:.1f} s; time_s extrapolates that rate to the full count"}
         out["graphs"][label] = rec
+        log(f"cpu baseline {label}: {rec}")
     first = next(iter(out["graphs"].values()))
     out["value"] = first["time_s"]
     out["sample"] = (f"{k}-clique, clique_test schedule, on the degree-oriented graph the GPU counts: "
@@ -213,9 +230,17 @@ PM6 = [("c6", "010001101000010100001010000101100010"), ("prism", "01110010101011
 MODEL_PATH = os.path.join(ROOT, "graphindor_amd", "data", "cost_model.gscm")
 FIXTURES = os.path.join(ROOT, "tests", "golden", "config_counts.json")
 # Root samples e % S == 0 of the pattern legs: those of the parity fixtures.
-PM_STRIDES = {"house": 4096, "c5": 4194304, "k4_pendant": 256, "c6": 4194304, "prism": 1048576}
-KNOBS = [{"lane_tail": 0, "hub_degree": 0}, {"lane_tail": -1, "hub_degree": 0}, {"lane_tail": 0, "hub_degree": -1},
-         {"lane_tail": -1, "hub_degree": -1}]
+PM_STRIDES = {"house": 4096, "c5": 4194304, "k4_pendant": 256, "c6": 33554432, "prism": 16777216}
+# The autotuned pattern-rule knobs: the lane-parallel last level on or off,
+# and hub bitmaps for vertices of degree >= 4096 or none (library default:
+# last level on, no hub bitmaps).
+HUB_DEGREE = 4096
+KNOBS = [{"lane_tail": 0, "hub_degree": -1}, {"lane_tail": -1, "hub_degree": -1},
+         {"lane_tail": 0, "hub_degree": HUB_DEGREE}, {"lane_tail": -1, "hub_degree": HUB_DEGREE}]
+
+
+def knob_label(kn):
+    return f"tail{'off' if kn['lane_tail'] < 0 else 'on'}_hub{'off' if kn['hub_degree'] < 0 else 'on'}"
 
 
 def fixture(key):
@@ -224,17 +249,18 @@ def fixture(key):
     return json.load(open(FIXTURES)).get(key)
 
 
-def pm_patterns(N, g, patterns, n, fix, budget_ms=60000):
+def pm_patterns(N, g, patterns, n, fix, budget_ms=60000, full=(), learned=True):
     """Each pattern on its fixed root sample: the plain analytic schedule with
     the library defaults (its raw sum and byte tally checked against the
     oracle's fixture), the four settings of the lane-parallel last level and
     the hub bitmaps (autotuning, on the same sample), and the schedule the
     learned cost model proposes and a measurement on the same sample
-    confirms."""
+    confirms.  Patterns named in `full` are then counted over every root
+    edge with the tuned knobs (stopped cleanly after 170 s)."""
     inf = g.info()
     v, e = (x.copy() for x in g.csr())
     dg = N.DeviceGraph(v, e)
-    model = N.CostModel.load(MODEL_PATH) if os.path.exists(MODEL_PATH) else None
+    model = N.CostModel.load(MODEL_PATH) if learned and os.path.exists(MODEL_PATH) else None
     out = {}
     for name, adj in patterns:
         S = PM_STRIDES[name]
@@ -249,10 +275,13 @@ def pm_patterns(N, g, patterns, n, fix, budget_ms=60000):
                 return None, None
         sa, ia = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
         pa = N.DevicePlan(sa.plan_desc())
+        log(f"pm {name}: {row['sampled_roots']} roots, plain analytic schedule (candidate {ia['analytic']} "
+            f"of {ia['candidates']})")
         raw, st = timed(pa, N.GM_COUNT_ALG_BYTES)
         if raw is None:
             row["plain_ms"] = f"over {budget_ms} ms"
             out[name] = row
+            log(f"pm {name}: over {budget_ms} ms")
             continue
         f = (fix or {}).get("patterns", {}).get(name)
         if f and f["stride"] == S:
@@ -262,17 +291,29 @@ def pm_patterns(N, g, patterns, n, fix, budget_ms=60000):
         for kn in KNOBS:
             r2, st2 = timed(pa, **kn)
             assert r2 == raw
-            times[f"tail{'off' if kn['lane_tail'] < 0 else 'on'}_hub{'off' if kn['hub_degree'] < 0 else 'on'}"] = \
-                round(st2.kernel_ms, 3)
-        row["plain_ms"] = times["tailon_hubon"]  # library defaults
+            times[knob_label(kn)] = round(st2.kernel_ms, 3)
+            log(f"pm {name}: analytic {knob_label(kn)} {st2.kernel_ms:.1f} ms")
+        row["plain_ms"] = times["tailon_huboff"]  # library defaults
+        best_knob = min(KNOBS, key=lambda kn: times[knob_label(kn)])
+        # The same knobs with 8 resident waves per CU, the build that keeps
+        # all its state in registers (the default runs 16, spilling some).
+        kn8 = dict(best_knob, waves_per_cu=8)
+        r8, st8 = timed(pa, **kn8)
+        assert r8 == raw
+        times[knob_label(best_knob) + "_w8"] = round(st8.kernel_ms, 3)
+        log(f"pm {name}: analytic {knob_label(best_knob)} at 8 waves per CU {st8.kernel_ms:.1f} ms")
+        if st8.kernel_ms < times[knob_label(best_knob)]:
+            best_knob = kn8
         row["analytic_knobs_ms"] = times
-        best_knob = min(KNOBS, key=lambda kn: times[f"tail{'off' if kn['lane_tail'] < 0 else 'on'}_hub{'off' if kn['hub_degree'] < 0 else 'on'}"])
+        row["tuned_knobs"] = knob_label(best_knob) + ("_w8" if "waves_per_cu" in best_knob else "")
         row["analytic_tuned_ms"] = min(times.values())
         row["analytic_candidate"] = ia["analytic"]
         row["candidates"] = ia["candidates"]
         if model is not None:
+            # Each candidate is cut off at twice the plain time.
             sl, il = N.select_schedule_measured(n, adj, g, model, root_stride=S,
-                                                time_limit_ms=int(max(200, 3 * row["plain_ms"])))
+                                                time_limit_ms=int(max(200, 2 * row["plain_ms"])))
+            log(f"pm {name}: learned selection chose candidate {il['chosen']} in {il['select_ms'] / 1e3:.1f} s")
             pl = N.DevicePlan(sl.plan_desc())
             r3, st3 = timed(pl)
             r4, st4 = timed(pl, **best_knob)
@@ -281,24 +322,35 @@ def pm_patterns(N, g, patterns, n, fix, budget_ms=60000):
                         "learned_tuned_ms": round(st4.kernel_ms, 3) if st4 else f"over {budget_ms} ms"})
             if st4:
                 row["learned_tuned_speedup_vs_plain"] = round(row["plain_ms"] / max(st4.kernel_ms, 1e-6), 3)
+        if name in full:
+            log(f"pm {name}: full count over {len(e)} root edges with {knob_label(best_knob)} "
+                f"(sample rate predicts {row['analytic_tuned_ms'] * S / 1e3:.0f} s)")
+            try:
+                rf, stf = N.count(dg, pa, time_limit_ms=170000, **best_knob)
+                row["full"] = {"count": rf // sa.redundancy, "raw": rf, "s": round(stf.kernel_ms / 1e3, 3),
+                               "knobs": knob_label(best_knob)}
+            except N.GmTimeout:
+                row["full"] = "over 170 s"
+            log(f"pm {name}: full count {row['full']}")
         out[name] = row
         log(f"pm {name}: {row}")
     dg.free()
     return out
 
 
-def pm_leg(N, scale=22, seed=4):
+def pm_leg(N, scale=22, seed=4, full=(), only=None, learned=True):
     """5-vertex pattern matching with IEP on R-MAT 2^scale ef16 seed 4 (house,
     5-cycle, 4-clique + pendant), each on the fixed root sample of its parity
     fixture.  Full counts at this size are beyond a bench run (the oracle's
     byte model puts the 5-cycle near 3e18 bytes of set work); the 4-clique +
-    pendant full count is timed by --pm-full."""
+    pendant full count (minutes) is timed when `full` names it."""
     t0 = time.time()
     g = N.Graph.rmat(scale, 16, seed)
     gen_s = time.time() - t0
     fix = fixture(f"rmat{scale}_s{seed}")
     out = {"graph": f"R-MAT 2^{scale} ef16 seed {seed}", "csr_entries": int(g.info()["e_cnt"]), "gen_s": round(gen_s, 1),
-           "patterns": pm_patterns(N, g, PM5, 5, fix)}
+           "patterns": pm_patterns(N, g, [p for p in PM5 if not only or p[0] in only], 5, fix, full=full,
+                                   learned=learned)}
     return out
 
 
@@ -311,23 +363,6 @@ def pm6_leg(N):
     fix = fixture("rmat23_ef32_s7")
     return {"graph": "R-MAT 2^23 ef32 seed 7", "csr_entries": int(g.info()["e_cnt"]), "gen_s": round(gen_s, 1),
             "patterns": pm_patterns(N, g, PM6, 6, fix)}
-
-
-def pm_full(N, scale=22, seed=4):
-    """4-clique + pendant on R-MAT 2^22 counted in full: plain defaults and
-    the autotuned settings, the count divided by the IEP redundancy."""
-    g = N.Graph.rmat(scale, 16, seed)
-    inf = g.info()
-    v, e = (x.copy() for x in g.csr())
-    dg = N.DeviceGraph(v, e)
-    s, _ = N.select_schedule(5, dict(PM5)["k4_pendant"], v, inf["tri_cnt"], None)
-    p = N.DevicePlan(s.plan_desc())
-    out = {}
-    for label, kn in (("plain", {}), ("tail_off", {"lane_tail": -1})):
-        raw, st = N.count(dg, p, **kn)
-        out[label] = {"count": raw // s.redundancy, "raw": raw, "s": round(st.kernel_ms / 1e3, 3)}
-        log(f"k4_pendant full count ({label}): {out[label]}")
-    return out
 
 
 def calibration_copy(mb):
@@ -386,9 +421,13 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched motif leg")
     ap.add_argument("--batch-stride", type=int, default=64)
     ap.add_argument("--no-pm", action="store_true", help="skip the 5-vertex pattern-matching leg")
-    ap.add_argument("--no-pm6", action="store_true", help="skip the 6-vertex (learned model) leg")
-    ap.add_argument("--pm-full", action="store_true", help="also count 4-clique + pendant on R-MAT 2^22 in full")
+    ap.add_argument("--pm6", action="store_true",
+                    help="also run the 6-vertex learned-model leg (R-MAT 2^23 ef32; minutes)")
+    ap.add_argument("--pm-full", action="store_true",
+                    help="also count 4-clique + pendant on R-MAT 2^22 in full (minutes)")
     ap.add_argument("--pm-scale", type=int, default=22)
+    ap.add_argument("--pm-patterns", default="", help="comma-separated subset of house,c5,k4_pendant")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-model selection in the PM legs")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC traffic calibration)")
     args = ap.parse_args()
@@ -398,7 +437,9 @@ def main():
         sys.exit(2)
 
     # The CPU baseline's OpenMP threads stay on neighbouring cores (read by
-    # libgomp when the host library first loads).
+    # libgomp when the host library first loads).  The host's CPUs are read
+    # before that: libgomp then binds this thread to one of them.
+    host = host_cpu()
     os.environ.setdefault("OMP_PROC_BIND", "close")
     from graphindor_amd import native as N
     if N.device_count() < 1:
@@ -527,14 +568,15 @@ def main():
     if not args.no_batch:
         out["config"]["batch_4motif"] = batch_leg(N, 512, args.batch_stride)
     if not args.no_pm:
-        out["config"]["pm_5vertex"] = pm_leg(N, args.pm_scale, 4)
-    if not args.no_pm6:
+        out["config"]["pm_5vertex"] = pm_leg(N, args.pm_scale, 4, full=("k4_pendant",) if args.pm_full else (),
+                                             only=[x for x in args.pm_patterns.split(",") if x],
+                                             learned=not args.no_learned)
+    if args.pm6:
         out["config"]["pm_6vertex_learned"] = pm6_leg(N)
-    if args.pm_full:
-        out["config"]["pm_full_k4_pendant"] = pm_full(N, args.pm_scale, 4)
     if not args.no_cpu:
         from oracle import oracle as O
-        out["cpu_baseline"] = cpu_baseline(N, O, [("degree_oriented", v, e), ("id_oriented", iv.copy(), ie.copy())], 5)
+        out["cpu_baseline"] = cpu_baseline(N, O, [("degree_oriented", v, e), ("id_oriented", iv.copy(), ie.copy())], 5,
+                                           host)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

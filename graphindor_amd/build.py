@@ -45,8 +45,9 @@ def _run(cmd, verbose):
 
 def build_device(verbose=False):
     """Compiles the device objects in parallel -- gm_engine.hip, gm_mlp.hip and
-    one object per enumerate_kernel<MODE, NMAX> instantiation of
-    enum_instance.hip -- and links libgm_hip.so."""
+    one object per enumerate_kernel<MODE, NMAX, WPE> instantiation of
+    enum_instance.hip, the deepest first since they take longest -- and links
+    libgm_hip.so."""
     from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
     obj_dir = os.path.join(LIB, "obj")
@@ -57,11 +58,16 @@ def build_device(verbose=False):
         base = os.path.splitext(os.path.basename(src))[0]
         if base == "enum_instance":
             for mode in (0, 1):
-                for nmax in (4, 5, 6, 7):
-                    jobs.append((src, os.path.join(obj_dir, f"{base}_{mode}_{nmax}.o"),
-                                 [f"-DGMK_MODE={mode}", f"-DGMK_NMAX={nmax}"]))
+                # the pattern rule in 2- and 4-wave-per-SIMD builds, the clique rule in the 2-wave one
+                for wpe in ((2, 4) if mode == 0 else (2,)):
+                    for nmax in (4, 5, 6, 7):
+                        jobs.append((src, os.path.join(obj_dir, f"{base}_{mode}_{nmax}_{wpe}.o"),
+                                     [f"-DGMK_MODE={mode}", f"-DGMK_NMAX={nmax}", f"-DGMK_WPE={wpe}"]))
         else:
             jobs.append((src, os.path.join(obj_dir, base + ".o"), []))
+    jobs.sort(key=lambda j: -sum(int(d.split("=")[1]) * (10 if "NMAX" in d else 0) + (5 if d == "-DGMK_MODE=0" else 0)
+                                 for d in j[2]))
+
     def compile_one(job):
         src, obj, defs = job
         _run([HIPCC, *HIP_FLAGS, *defs, f"-I{INC}", f"-I{dev}", "-c", src, "-o", obj], verbose)

@@ -87,21 +87,26 @@ struct Counters {
     alignas(128) unsigned long long qhead;  // slice queue: next slot to pop
     unsigned long long qtail;      // slice queue: slots reserved by pushers
     unsigned long long done_items; // (root, plan) items fully processed, donations included
+    unsigned long long slices_done; // queued slices fully processed, their own donations included
     unsigned long long donations;  // donate_slices calls that queued slices
+    unsigned long long deep_donations; // of those, donations of depth-3 slices
     unsigned long long retries;    // queue-head CAS attempts lost to another popper
     unsigned long long slice_ticks; // wall-clock ticks waves spent on donated slices (sum over waves)
     unsigned long long idle_ticks;  // wall-clock ticks waves spent waiting for slices (sum over waves)
 };
 
-// A slice of one root edge's depth-2 loop set, donated by the wave that
-// found the loop set long: candidates [s0, s1) of root edge e under plan pl
-// of graph g.  `ready` is set last by the pusher and cleared by the popper,
-// so the queue is all-clear between launches.
+// A slice of one root edge's work, donated by the wave that found a loop
+// set long: candidates [s0, s1) of the depth-2 loop set of root edge e under
+// plan pl of graph g and, below each of them, candidates [t0, t1) of the
+// depth-3 loop set.  A depth-2 slice has t0 = 0, t1 = kAllCandidates; a
+// depth-3 slice has s1 = s0 + 1 (one depth-2 candidate) and t0 > 0.
+// `ready` is set last by the pusher and cleared by the popper, so the queue
+// is all-clear between launches.
+constexpr int32_t kAllCandidates = 0x7fffffff;
 struct SliceItem {
     int64_t e;
-    int32_t pl, g, s0, s1;
+    int32_t pl, g, s0, s1, t0, t1;
     int32_t ready;
-    int32_t pad;
 };
 
 struct KernelArgs {
@@ -132,6 +137,7 @@ struct KernelArgs {
     int64_t queue_cap;
     int32_t slice_len;        // depth-2 candidates per donated slice
     int32_t eager_donation;   // 1: donate long remainders at once, not only when roots run out
+    int32_t deep_donation;    // 1: depth-3 loop sets are donated too, and slices count themselves done
     int64_t time_limit_ticks; // wall-clock ticks a wave may run (0: no limit)
     const unsigned long long* hub_bits;  // neighbour bitmaps of the hub vertices (null: none)
     const int32_t* hub_slot;             // per vertex: index of its bitmap, or -1
@@ -223,14 +229,22 @@ struct Wave {
     unsigned long long lane_bytes; // per lane
     unsigned int overflow;
     int32_t slice_lo, slice_hi;    // depth-2 candidates this wave walks for the current root
+    int32_t s3_lo, s3_hi;          // depth-3 candidates it walks below each of them
+    int32_t deep;                  // 1: the current item is a depth-3 slice (its depth-2 candidate's
+                                   // prefixes are rebuilt and charged by the donor)
+    int32_t stats0;                // the launch's stats flag (a.stats is cleared during rebuilds)
     int32_t donate;                // 1: long depth-2 loop sets are split and donated
+    int32_t donate3;               // 1: long depth-3 loop sets are split and donated
+    int32_t cur_i2;                // index of the current depth-2 candidate in its loop set
     int32_t cur_pl, cur_gr;        // plan and graph of the current root (for donated slices)
     int64_t cur_e;                 // current root edge
     int64_t nitems;                // (root, plan) work items of the launch
     uint64_t t_start;              // wall clock at the wave's start
     int32_t aborted;               // 1: the time limit passed; unwind
     int64_t rebuild_ticks;         // wall-clock ticks the current root's depth-0/1 builds took
+    int64_t rebuild2_ticks;        // wall-clock ticks the current depth-2 candidate's builds took
     int32_t donations;             // donations this wave made
+    int32_t deep_donations;        // of those, of depth-3 slices
 };
 
 __device__ __forceinline__ SetRef get_set(const Wave& w, int pid) {
@@ -685,6 +699,7 @@ __device__ __forceinline__ void tail_candidate(const Wave& w, const TailCtx& c, 
 
 template <int D>
 __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t bound);
+template <int D>
 __device__ __forceinline__ int32_t maybe_donate(Wave& w, int32_t lo, int32_t base, int32_t hi, uint64_t t_loop0,
                                                 int32_t step_mask);
 
@@ -771,18 +786,19 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     const int ln = lane_id();
     const TailCtx& c = w.lds->tail;
     unsigned long long acc = 0, bytes = 0;
-    // At depth 2 a root's candidates may be split and donated exactly as in
-    // the DFS loop: every candidate's contribution is its own.
+    // At depths 2 and 3 a root's candidates may be split and donated exactly
+    // as in the DFS loop: every candidate's contribution is its own.
     int32_t lo = 0, hi = loop.len;
     uint64_t t_loop0 = 0;
-    if constexpr (D == 2) {
-        lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
-        hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
-        if (w.donate) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
+    if constexpr (D == 2 || D == 3) {
+        const int32_t l0 = D == 2 ? w.slice_lo : w.s3_lo, h0 = D == 2 ? w.slice_hi : w.s3_hi;
+        lo = l0 < loop.len ? l0 : loop.len;
+        hi = h0 < loop.len ? h0 : loop.len;
+        if (D == 2 ? w.donate : w.donate3) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
     }
     for (int32_t b0 = lo; b0 < hi; b0 += kWave) {
         if (out_of_time(w)) break;
-        if constexpr (D == 2) hi = maybe_donate(w, lo, b0, hi, t_loop0, 15);
+        if constexpr (D == 2 || D == 3) hi = maybe_donate<D>(w, lo, b0, hi, t_loop0, 15);
         const int32_t i = b0 + ln;
         const bool in_range = i < hi;
         const uint32_t v = in_range ? loop.ptr[i] : 0xffffffffu;
@@ -824,9 +840,11 @@ __device__ __forceinline__ void pattern_tail_loop(Wave& w, SetRef loop, uint32_t
     wave_sync();
 }
 
-// Pushes slices [lo + T, hi) of the current root's depth-2 loop set, T
-// candidates each, and returns the end of the part this wave keeps.  Slots
-// past the queue's capacity are kept too.
+// Pushes slices [lo + T, hi) of the current root's depth-D loop set (D = 2,
+// or 3 below the current depth-2 candidate), T candidates each, and returns
+// the end of the part this wave keeps.  Slots past the queue's capacity are
+// kept too.
+template <int D>
 __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi, int32_t T) {
     const int32_t keep_hi = lo + T;
     const int32_t nslices = (hi - keep_hi + T - 1) / T;
@@ -836,7 +854,10 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
     const int64_t cap = w.a.queue_cap;
     const int32_t queued = base >= (unsigned long long)cap ? 0
                            : (int32_t)(((int64_t)base + nslices <= cap) ? nslices : cap - (int64_t)base);
-    if (queued > 0) ++w.donations;
+    if (queued > 0) {
+        ++w.donations;
+        if (D == 3) ++w.deep_donations;
+    }
     // The last `queued` slices go to the queue; the ones before them that
     // did not fit stay with this wave, so its part remains contiguous.
     const int32_t first = nslices - queued;
@@ -846,8 +867,18 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
         it->e = w.cur_e;
         it->pl = w.cur_pl;
         it->g = w.cur_gr;
-        it->s0 = keep_hi + k * T;
-        it->s1 = keep_hi + (k + 1) * T < hi ? keep_hi + (k + 1) * T : hi;
+        const int32_t c0 = keep_hi + k * T, c1 = keep_hi + (k + 1) * T < hi ? keep_hi + (k + 1) * T : hi;
+        if (D == 2) {
+            it->s0 = c0;
+            it->s1 = c1;
+            it->t0 = 0;
+            it->t1 = kAllCandidates;
+        } else {
+            it->s0 = w.cur_i2;
+            it->s1 = w.cur_i2 + 1;
+            it->t0 = c0;
+            it->t1 = c1;
+        }
     }
     // One release fence (an L2 write-back) for the whole donation orders
     // every lane's item stores before its `ready` flags.
@@ -858,34 +889,47 @@ __device__ __forceinline__ int32_t donate_slices(Wave& w, int32_t lo, int32_t hi
     return kept < hi ? kept : hi;
 }
 
-// Donation check of a depth-2 loop at candidate `base` of [lo, hi): once no
-// root is left to claim, a remainder that would keep this wave busy is cut
-// into slices for idle waves; returns the new end of this wave's part.
-// Slice length from this root's own costs: a slice's work (T candidates at
-// the per-candidate time measured since t_loop0) is at least 8x the prefix
-// rebuild and claim a popper pays for it.  The check runs at the 64-candidate
-// steps whose index has no bit of step_mask set (the lane tail walks a step in
-// about a microsecond, so it checks every 16th).  Wave-uniform.
+// Donation check of a depth-D loop (D = 2 or 3) at candidate `base` of
+// [lo, hi): once no root is left to claim, a remainder that would keep this
+// wave busy is cut into slices for idle waves; returns the new end of this
+// wave's part.  Depth 3 donates only while waves wait on an empty queue:
+// depth-2 slices come first, and a depth-3 slice costs its popper the
+// depth-2 candidate's builds as well.  Slice length from this root's own
+// costs: a slice's work (T candidates at the per-candidate time measured
+// since t_loop0) is at least 8x the prefix rebuild and claim a popper pays
+// for it.  The check runs at the 64-candidate steps whose index has no bit
+// of step_mask set (the lane tail walks a step in about a microsecond, so it
+// checks every 16th).  Wave-uniform.
+template <int D>
 __device__ __forceinline__ int32_t maybe_donate(Wave& w, int32_t lo, int32_t base, int32_t hi, uint64_t t_loop0,
                                                 int32_t step_mask) {
-    if (!w.donate || hi - base <= 4 * w.a.slice_len) return hi;
+    if (!(D == 2 ? w.donate : w.donate3) || hi - base <= 4 * w.a.slice_len) return hi;
     if ((((base - lo) / kWave) & step_mask) != 0) return hi;
     int32_t T = w.a.slice_len;
     bool go = w.a.eager_donation != 0;
     if (!go && base > lo) {
-        unsigned long long c = 0;
-        if (lane_id() == 0) c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long c = 0, wait = 1;
+        if (lane_id() == 0) {
+            c = __hip_atomic_load(&w.a.ctr->cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (D == 3) {
+                const unsigned long long qh = __hip_atomic_load(&w.a.ctr->qhead, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long qt = __hip_atomic_load(&w.a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                wait = qh > qt ? 1 : 0;
+            }
+        }
         c = (unsigned long long)uni64(__shfl((long long)c, 0));
-        if ((int64_t)c >= w.nitems) {
+        wait = (unsigned long long)uni64(__shfl((long long)wait, 0));
+        if ((int64_t)c >= w.nitems && wait) {
             const int64_t el = uni64((int64_t)wall_clock64()) - (int64_t)t_loop0;
             const int64_t per = el / (base - lo) + 1;  // ticks per candidate
             const int64_t rest = per * (int64_t)(hi - base);
-            const int64_t want = (8 * (w.rebuild_ticks + kSliceClaimTicks)) / per + 1;
+            const int64_t rebuild = w.rebuild_ticks + (D == 3 ? w.rebuild2_ticks : 0);
+            const int64_t want = (8 * (rebuild + kSliceClaimTicks)) / per + 1;
             T = (int32_t)(want < T ? T : (want > (hi - base) / 2 ? (hi - base) / 2 : want));
             go = rest > kDonateMinTicks && hi - base > 2 * T;
         }
     }
-    return go ? donate_slices(w, base, hi, T) : hi;
+    return go ? donate_slices<D>(w, base, hi, T) : hi;
 }
 
 template <int D, int MODE, int NMAX>
@@ -916,25 +960,29 @@ __device__ __forceinline__ void dfs(Wave& w) {
             clique_tail_count(w, loop);
             return;
         }
-        if (MODE == GM_MODE_PATTERN && D == w.tail_depth) {
-            pattern_tail<D>(w, loop);
-            return;
+        // The tail sits at depth n-2 at most (a plan has a last level below it),
+        // so depths from GM_MAX_PATTERN - 1 on never instantiate it.
+        if constexpr (D + 1 < GM_MAX_PATTERN) {
+            if (MODE == GM_MODE_PATTERN && D == w.tail_depth) {
+                pattern_tail<D>(w, loop);
+                return;
+            }
         }
         const uint32_t bound = (MODE == GM_MODE_PATTERN) ? restriction_bound<D>(w) : 0xffffffffu;
         const int ln = lane_id();
         int32_t lo = 0, hi = loop.len;
-        if constexpr (D == 2) {
-            // A root's depth-2 candidates may be split: this wave walks
-            // [slice_lo, slice_hi) (a donated slice, or the whole set).
-            lo = w.slice_lo < loop.len ? w.slice_lo : loop.len;
-            hi = w.slice_hi < loop.len ? w.slice_hi : loop.len;
-        }
         uint64_t t_loop0 = 0;
-        if constexpr (D == 2) {
-            if (w.donate) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
+        if constexpr (D == 2 || D == 3) {
+            // A root's depth-2 and depth-3 candidates may be split: this wave
+            // walks [slice_lo, slice_hi) at depth 2 and [s3_lo, s3_hi) at
+            // depth 3 (a donated slice, or the whole sets).
+            const int32_t l0 = D == 2 ? w.slice_lo : w.s3_lo, h0 = D == 2 ? w.slice_hi : w.s3_hi;
+            lo = l0 < loop.len ? l0 : loop.len;
+            hi = h0 < loop.len ? h0 : loop.len;
+            if (D == 2 ? w.donate : w.donate3) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
         }
         for (int32_t base = lo; base < hi; base += kWave) {
-            if constexpr (D == 2) hi = maybe_donate(w, lo, base, hi, t_loop0, 0);
+            if constexpr (D == 2 || D == 3) hi = maybe_donate<D>(w, lo, base, hi, t_loop0, 0);
             int32_t i = base + ln;
             uint32_t cand = 0xffffffffu;
             int64_t rb = 0, re = 0;
@@ -948,6 +996,13 @@ __device__ __forceinline__ void dfs(Wave& w) {
                 uint32_t v = rdlane(cand, j);
                 if (v >= bound) return;  // sorted: nothing further qualifies
                 if (out_of_time(w)) return;
+                uint64_t t_b2 = 0;
+                if constexpr (D == 2) {
+                    // A depth-3 slice rebuilds its one depth-2 candidate's
+                    // prefixes, which the donor has charged.
+                    w.a.stats = w.deep ? 0 : w.stats0;
+                    if (w.donate3) t_b2 = (uint64_t)uni64((int64_t)wall_clock64());
+                }
                 if (w.a.stats) w.bytes += 4ull;
                 if (MODE == GM_MODE_PATTERN && in_emb<D>(w, v)) continue;
                 int64_t l = rdlane64(rb, j), r = rdlane64(re, j);
@@ -960,6 +1015,11 @@ __device__ __forceinline__ void dfs(Wave& w) {
                         prune = true;
                         break;
                     }
+                }
+                if constexpr (D == 2) {
+                    w.a.stats = w.stats0;
+                    if (w.donate3) w.rebuild2_ticks = uni64((int64_t)wall_clock64()) - (int64_t)t_b2;
+                    w.cur_i2 = base + j;
                 }
                 if (prune) continue;
                 w.emb[D] = v;
@@ -986,7 +1046,7 @@ template <int MODE, int NMAX>
 // root's prefixes without charging them again: the wave that found the root
 // charged them.
 __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, int32_t lo, int32_t hi,
-                                             bool donated) {
+                                             bool donated, int32_t lo3 = 0, int32_t hi3 = kAllCandidates) {
     const DevPlan& p = *w.pl;
     const int32_t stats = w.a.stats;
     if (donated) w.a.stats = 0;
@@ -1009,8 +1069,13 @@ __device__ __forceinline__ void process_root(Wave& w, uint32_t v0, uint32_t v1, 
     w.emb[1] = v1;
     w.slice_lo = lo;
     w.slice_hi = hi;
+    w.s3_lo = lo3;
+    w.s3_hi = hi3;
+    w.deep = (lo3 != 0 || hi3 != kAllCandidates) ? 1 : 0;
     w.donate = (!donated && w.a.queue != nullptr) ? 1 : 0;
+    w.donate3 = (w.a.deep_donation && !w.deep && w.a.queue != nullptr) ? 1 : 0;
     dfs<2, MODE, NMAX>(w);
+    w.a.stats = w.stats0;
 }
 
 // Claims the next pushed slice without waiting (lane 0 CAS on the queue head;
@@ -1073,9 +1138,14 @@ __device__ __forceinline__ void flush_sum(Wave& w, int32_t g, int32_t pi) {
 // in cache.  A root's count goes to the (graph, plan) cell of its edge.
 // NMAX: the largest pattern size among the launch's plans; the DFS is
 // instantiated only that deep, which keeps registers (and so occupancy) to
-// what the patterns need.
-template <int MODE, int NMAX>
-__global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
+// what the patterns need.  WPE: waves per SIMD the build must fit.  At 2 the
+// pattern-rule DFS keeps all its state in registers (about 230 VGPRs); at 4
+// it is held to 128 VGPRs and spills some 400 bytes per lane to scratch,
+// which hides more of the latency of its scattered list reads: on R-MAT
+// 2^22 house and 4-clique + pendant ran 1.46x and 1.49x faster, the 5-cycle
+// (31 roots, bound by its few heaviest slices) 1.22x slower.
+template <int MODE, int NMAX, int WPE>
+__global__ void __launch_bounds__(kBlock, WPE) enumerate_kernel(KernelArgs a) {
     __shared__ WaveShared<MODE> lds_all[kWavesPerBlock];
     Wave w;
     w.a = a;
@@ -1090,6 +1160,15 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     w.lane_bytes = 0;
     w.overflow = 0;
     w.aborted = 0;
+    w.stats0 = a.stats;
+    w.deep = 0;
+    w.donate = 0;
+    w.donate3 = 0;
+    w.cur_i2 = 0;
+    w.s3_lo = 0;
+    w.s3_hi = kAllCandidates;
+    w.rebuild_ticks = 0;
+    w.rebuild2_ticks = 0;
     w.t_start = (uint64_t)uni64((int64_t)wall_clock64());
 #pragma unroll
     for (int i = 0; i <= GM_MAX_PATTERN; ++i) w.emb[i] = 0;
@@ -1118,6 +1197,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
     // Queue diagnostics (lane 0's CAS losses; wave-uniform tick sums).
     unsigned long long lost = 0, slice_ticks = 0, idle_ticks = 0;
     w.donations = 0;
+    w.deep_donations = 0;
     while (true) {
         if (out_of_time(w)) break;
         // Donated slices first: they belong to roots already under way.
@@ -1130,11 +1210,15 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 // An idle wave holds a ticket: the next queue slot, taken
                 // with one fetch-add (a CAS claim costs one attempt per
                 // contending wave, and thousands of waves idle at the end).
-                // It waits until its slot is pushed, or leaves once every
-                // root item is done -- only roots push, so the tail is then
-                // final -- and the ticket lies past the tail.  (Counting
-                // items, not waves, keeps the rule independent of how many
-                // of the grid's waves are resident.)
+                // It waits until its slot is pushed, or leaves once nothing
+                // can push any more and the ticket lies past the tail.  Only
+                // roots push unless depth-3 donation is on, so then nothing
+                // can once every root item is done; with depth-3 donation
+                // depth-2 slices push too, so also every queued slot must
+                // have been processed (slices_done == min(qtail, cap): a
+                // slot in flight is queued but not done).  Counting items,
+                // not waves, keeps the rule independent of how many of the
+                // grid's waves are resident.
                 if (ticket < 0) {
                     unsigned long long t = 0;
                     if (ln == 0) t = atomicAdd(&a.ctr->qhead, 1ull);
@@ -1152,7 +1236,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                     if (ln == 0)
                         fin = __hip_atomic_load(&a.ctr->done_items, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     fin = (unsigned long long)uni64(__shfl((long long)fin, 0));
-                    if ((int64_t)fin >= nitems) {
+                    if ((int64_t)fin >= nitems && !a.deep_donation) {
                         // Pairs with the release adds of done_items: every
                         // push reserved its slots before its root's add.
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1160,6 +1244,35 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                         tail = (unsigned long long)uni64(__shfl((long long)tail, 0));
                         if ((unsigned long long)ticket >= tail) break;
                         continue;  // pushed meanwhile: the next pass takes it
+                    }
+                    if ((int64_t)fin >= nitems) {
+                        // Every root is done; the tail is final once every
+                        // queued slot is done too.  The relaxed reads gate the
+                        // one acquire (an L2 invalidate on this XCD, which an
+                        // idle wave must not repeat under the waves still
+                        // walking slices): only when they say so is the
+                        // acquire taken, pairing with the release adds of
+                        // done_items and slices_done (every push reserved its
+                        // slots before its root's or slice's add), and the
+                        // tail read again after it.  slices_done is read
+                        // before that tail, so a slice still in flight keeps
+                        // the count below the tail read after it.
+                        unsigned long long sd = 0;
+                        if (ln == 0)
+                            sd = __hip_atomic_load(&a.ctr->slices_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        sd = (unsigned long long)uni64(__shfl((long long)sd, 0));
+                        const unsigned long long seen = tail < (unsigned long long)a.queue_cap
+                                                            ? tail : (unsigned long long)a.queue_cap;
+                        if (sd >= seen) {
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                            if (ln == 0)
+                                tail = __hip_atomic_load(&a.ctr->qtail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            tail = (unsigned long long)uni64(__shfl((long long)tail, 0));
+                            const unsigned long long queued = tail < (unsigned long long)a.queue_cap
+                                                                  ? tail : (unsigned long long)a.queue_cap;
+                            if ((unsigned long long)ticket < tail) continue;  // pushed meanwhile: the next pass takes it
+                            if (sd >= queued) break;  // the tail is final
+                        }
                     }
                     // Back off exponentially; relaxed polls, nothing is read
                     // on their strength (an acquire would invalidate this
@@ -1186,6 +1299,8 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 const int32_t gj = unii(__shfl(it->g, 0));
                 const int32_t s0 = unii(__shfl(it->s0, 0));
                 const int32_t s1 = unii(__shfl(it->s1, 0));
+                const int32_t t0 = unii(__shfl(it->t0, 0));
+                const int32_t t1 = unii(__shfl(it->t1, 0));
                 if (ln == 0) __hip_atomic_store(&it->ready, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (gj != cur_g || pj != cur_p) {
                     flush_sum(w, cur_g, cur_p);
@@ -1196,7 +1311,14 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
                 w.tail_depth = a.tail_off ? -1 : w.pl->tail_depth;
                 const uint32_t u1 = a.col[e];
                 const uint32_t u0 = (uint32_t)edge_source(a.row, a.nv, e);
-                process_root<MODE, NMAX>(w, u0, u1, s0, s1, true);
+                // A depth-2 slice may donate depth-3 slices of its own root.
+                w.cur_e = e;
+                w.cur_pl = pj;
+                w.cur_gr = gj;
+                process_root<MODE, NMAX>(w, u0, u1, s0, s1, true, t0, t1);
+                // Release: this slice's own pushes precede it.
+                if (a.deep_donation && ln == 0 && !w.aborted)
+                    __hip_atomic_fetch_add(&a.ctr->slices_done, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 slice_ticks += uni64((int64_t)wall_clock64()) - (int64_t)t_pop;
                 idle_sleeps = 1;
                 continue;
@@ -1274,6 +1396,7 @@ __global__ void __launch_bounds__(kBlock) enumerate_kernel(KernelArgs a) {
         if (w.aborted) atomicOr(&a.ctr->timed_out, 1u);
         if (a.queue) {
             if (w.donations) atomicAdd(&a.ctr->donations, (unsigned long long)w.donations);
+            if (w.deep_donations) atomicAdd(&a.ctr->deep_donations, (unsigned long long)w.deep_donations);
             if (lost) atomicAdd(&a.ctr->retries, lost);
             if (slice_ticks) atomicAdd(&a.ctr->slice_ticks, slice_ticks);
             if (idle_ticks) atomicAdd(&a.ctr->idle_ticks, idle_ticks);
@@ -1899,19 +2022,24 @@ __global__ void __launch_bounds__(kWave) clique_bitmap_small_kernel(BmArgs a) {
     }
 }
 
-// One instantiation of enumerate_kernel<MODE, NMAX> per object file
-// (enum_instance.hip); the host half dispatches on (mode, nmax).
-#define GMK_ENUM_DECL(M, N)                                                         \
-    hipError_t enum_occupancy_##M##_##N(int* blocks_per_cu);                        \
-    void enum_launch_##M##_##N(int blocks, hipStream_t s, const KernelArgs& a);
-GMK_ENUM_DECL(0, 4)
-GMK_ENUM_DECL(0, 5)
-GMK_ENUM_DECL(0, 6)
-GMK_ENUM_DECL(0, 7)
-GMK_ENUM_DECL(1, 4)
-GMK_ENUM_DECL(1, 5)
-GMK_ENUM_DECL(1, 6)
-GMK_ENUM_DECL(1, 7)
+// One instantiation of enumerate_kernel<MODE, NMAX, WPE> per object file
+// (enum_instance.hip); the host half dispatches on (mode, nmax, wpe).  The
+// pattern rule is built for 2 and 4 waves per SIMD, the clique rule for 2.
+#define GMK_ENUM_DECL(M, N, W)                                                           \
+    hipError_t enum_occupancy_##M##_##N##_##W(int* blocks_per_cu);                       \
+    void enum_launch_##M##_##N##_##W(int blocks, hipStream_t s, const KernelArgs& a);
+GMK_ENUM_DECL(0, 4, 2)
+GMK_ENUM_DECL(0, 5, 2)
+GMK_ENUM_DECL(0, 6, 2)
+GMK_ENUM_DECL(0, 7, 2)
+GMK_ENUM_DECL(0, 4, 4)
+GMK_ENUM_DECL(0, 5, 4)
+GMK_ENUM_DECL(0, 6, 4)
+GMK_ENUM_DECL(0, 7, 4)
+GMK_ENUM_DECL(1, 4, 2)
+GMK_ENUM_DECL(1, 5, 2)
+GMK_ENUM_DECL(1, 6, 2)
+GMK_ENUM_DECL(1, 7, 2)
 #undef GMK_ENUM_DECL
 
 }  // namespace gmk

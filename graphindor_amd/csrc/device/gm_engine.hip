@@ -90,23 +90,54 @@ int fail(int code, const std::string& msg) {
 // Kernel instantiation for a launch whose largest pattern has n vertices.
 int nmax_for(int n) { return n <= 4 ? 4 : n; }
 
+// Depth-3 donation (GM_COUNT_DEEP_DONATION) is refused until its stall on
+// hub-heavy graphs is understood: a 5-cycle sample of R-MAT 2^22 that takes
+// 8 s without it ran past 180 s with it.  The kernels keep the code.
+const char* const kDeepDonationOff =
+    "GM_COUNT_DEEP_DONATION is switched off: depth-3 donation stalled on hub-heavy graphs";
+
+// Waves per SIMD of the build a launch uses: the pattern rule runs the
+// 4-wave build (128 VGPRs, some state in scratch) unless the tuning asks for
+// at most 8 resident waves per CU, which the 2-wave build (all state in
+// registers) holds; the clique rule has the 2-wave build only.
+int enum_wpe(int mode, int req_waves) {
+    return (mode == GM_MODE_PATTERN && (req_waves <= 0 || req_waves > 8)) ? 4 : 2;
+}
+
 template <int MODE>
-hipError_t enum_occupancy(int nmax, int* blocks_per_cu) {
+hipError_t enum_occupancy(int nmax, int wpe, int* blocks_per_cu) {
+    if (MODE == 0 && wpe == 4) {
+        switch (nmax) {
+            case 4: return enum_occupancy_0_4_4(blocks_per_cu);
+            case 5: return enum_occupancy_0_5_4(blocks_per_cu);
+            case 6: return enum_occupancy_0_6_4(blocks_per_cu);
+            default: return enum_occupancy_0_7_4(blocks_per_cu);
+        }
+    }
     switch (nmax) {
-        case 4: return MODE == 0 ? enum_occupancy_0_4(blocks_per_cu) : enum_occupancy_1_4(blocks_per_cu);
-        case 5: return MODE == 0 ? enum_occupancy_0_5(blocks_per_cu) : enum_occupancy_1_5(blocks_per_cu);
-        case 6: return MODE == 0 ? enum_occupancy_0_6(blocks_per_cu) : enum_occupancy_1_6(blocks_per_cu);
-        default: return MODE == 0 ? enum_occupancy_0_7(blocks_per_cu) : enum_occupancy_1_7(blocks_per_cu);
+        case 4: return MODE == 0 ? enum_occupancy_0_4_2(blocks_per_cu) : enum_occupancy_1_4_2(blocks_per_cu);
+        case 5: return MODE == 0 ? enum_occupancy_0_5_2(blocks_per_cu) : enum_occupancy_1_5_2(blocks_per_cu);
+        case 6: return MODE == 0 ? enum_occupancy_0_6_2(blocks_per_cu) : enum_occupancy_1_6_2(blocks_per_cu);
+        default: return MODE == 0 ? enum_occupancy_0_7_2(blocks_per_cu) : enum_occupancy_1_7_2(blocks_per_cu);
     }
 }
 
 template <int MODE>
-void enum_launch(int nmax, int blocks, hipStream_t s, const KernelArgs& a) {
+void enum_launch(int nmax, int wpe, int blocks, hipStream_t s, const KernelArgs& a) {
+    if (MODE == 0 && wpe == 4) {
+        switch (nmax) {
+            case 4: enum_launch_0_4_4(blocks, s, a); break;
+            case 5: enum_launch_0_5_4(blocks, s, a); break;
+            case 6: enum_launch_0_6_4(blocks, s, a); break;
+            default: enum_launch_0_7_4(blocks, s, a); break;
+        }
+        return;
+    }
     switch (nmax) {
-        case 4: MODE == 0 ? enum_launch_0_4(blocks, s, a) : enum_launch_1_4(blocks, s, a); break;
-        case 5: MODE == 0 ? enum_launch_0_5(blocks, s, a) : enum_launch_1_5(blocks, s, a); break;
-        case 6: MODE == 0 ? enum_launch_0_6(blocks, s, a) : enum_launch_1_6(blocks, s, a); break;
-        default: MODE == 0 ? enum_launch_0_7(blocks, s, a) : enum_launch_1_7(blocks, s, a); break;
+        case 4: MODE == 0 ? enum_launch_0_4_2(blocks, s, a) : enum_launch_1_4_2(blocks, s, a); break;
+        case 5: MODE == 0 ? enum_launch_0_5_2(blocks, s, a) : enum_launch_1_5_2(blocks, s, a); break;
+        case 6: MODE == 0 ? enum_launch_0_6_2(blocks, s, a) : enum_launch_1_6_2(blocks, s, a); break;
+        default: MODE == 0 ? enum_launch_0_7_2(blocks, s, a) : enum_launch_1_7_2(blocks, s, a); break;
     }
 }
 
@@ -139,12 +170,17 @@ struct DeviceCache {
     SliceItem* queue = nullptr;      // donated depth-2 slices (ready flags all-clear between launches)
 };
 
-constexpr int64_t kQueueCap = 1 << 20;  // 32 MiB of slice items per device
+// Slots of the slice queue, each used once per launch: 160 MiB of items per
+// device.  At 2^20 slots the house sample of R-MAT 2^22 (1/4096 of the
+// roots) pushed 1,023,669 slices, and with 1.5x the waves it filled the
+// queue, after which idle waves leave and the rest ran 4x slower.
+constexpr int64_t kQueueCap = 1 << 22;
 
 // The slice queue's counters of a list-engine launch (wall clock: 100 MHz).
 void fill_queue_stats(const Counters& c, gm_stats* st) {
     st->slices = c.qtail < (unsigned long long)kQueueCap ? c.qtail : (unsigned long long)kQueueCap;
     st->donations = c.donations;
+    st->deep_donations = c.deep_donations;
     st->queue_retries = c.retries;
     st->slice_wave_ms = (double)c.slice_ticks * 1e-5;
     st->idle_wave_ms = (double)c.idle_ticks * 1e-5;
@@ -355,7 +391,12 @@ struct gm_graph {
 
 namespace {
 
-constexpr int32_t kDefaultHubDegree = 4096;
+// Hub bitmaps are off unless asked for: a bitmap over vertex ids turns a
+// binary search in a hub's list into one load, but its lines are scattered
+// over n/8 bytes, and on R-MAT 2^22 the house pattern ran 2.7x slower with
+// them while the 4-clique + pendant ran 1.38x faster.  A tuner that measures
+// both picks the threshold per pattern.
+constexpr int32_t kDefaultHubDegree = 0;
 constexpr size_t kHubBytesCap = (size_t)4 << 30;
 
 // Builds (or reuses) the hub bitmaps of graph g for threshold T on stream s.
@@ -678,6 +719,8 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
     if (!g || !plans || np <= 0 || !raw_sums || ng < 1) return fail(GM_ERR_INVALID, "gm_count: bad arguments");
     for (int i = 0; i < np; ++i)
         if (!plans[i]) return fail(GM_ERR_INVALID, "gm_count: null plan");
+    if (flags & GM_COUNT_DEEP_DONATION)
+        return fail(GM_ERR_UNSUPPORTED, kDeepDonationOff);
     if (graph_edge_offsets) {
         if (graph_edge_offsets[0] < 0 || graph_edge_offsets[ng] > g->nnz)
             return fail(GM_ERR_INVALID, "gm_count_batch: edge offsets outside [0, nnz]");
@@ -703,15 +746,18 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
         m = std::max(m, nmax_for(plans[i]->host.n));
     }
     const int req_waves = (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0;
+    // The static workspace runs the 2-wave build: the 4-wave build's spill
+    // space is private scratch the runtime would have to provide.
+    const int wpe_pat = fixed_ws ? 2 : enum_wpe(GM_MODE_PATTERN, req_waves);
     int blocks_pat = 0, blocks_clq = 0;
     if (nmax_pat) {
         int occ = 0;
-        HIP_TRY(enum_occupancy<GM_MODE_PATTERN>(nmax_pat, &occ));
+        HIP_TRY(enum_occupancy<GM_MODE_PATTERN>(nmax_pat, wpe_pat, &occ));
         blocks_pat = grid_blocks(cus, occ, req_waves);
     }
     if (nmax_clq) {
         int occ = 0;
-        HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, &occ));
+        HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, enum_wpe(GM_MODE_CLIQUE, req_waves), &occ));
         blocks_clq = grid_blocks(cus, occ, req_waves);
     }
     int64_t nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
@@ -883,6 +929,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
             a.queue_cap = kQueueCap;
             a.slice_len = sl;
             a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
+        a.deep_donation = (flags & GM_COUNT_DEEP_DONATION) ? 1 : 0;
         }
     }
     if (t && t->time_limit_ms > 0) {
@@ -902,7 +949,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
         a.sums = d_sums;
         a.root_min_outdeg = 0;
         HIP_TRY(rec(dc.evpool.empty() ? nullptr : dc.evpool[0]));
-        enum_launch<GM_MODE_PATTERN>(nmax_pat, blocks_pat, s, a);
+        enum_launch<GM_MODE_PATTERN>(nmax_pat, wpe_pat, blocks_pat, s, a);
         HIP_TRY(hipGetLastError());
         HIP_TRY(rec(dc.evpool.empty() ? nullptr : dc.evpool[1]));
         ev_list.push_back(0);
@@ -1016,7 +1063,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
         // bytes, no task -- so the launch, a 9.5 ms scan there, is skipped.
         if (!(use_bitmap && g->max_deg <= bm_max)) {
             HIP_TRY(rec(timed ? dc.evpool[evl] : nullptr));
-            enum_launch<GM_MODE_CLIQUE>(nmax_clq, blocks_clq, s, a);
+            enum_launch<GM_MODE_CLIQUE>(nmax_clq, enum_wpe(GM_MODE_CLIQUE, req_waves), blocks_clq, s, a);
             HIP_TRY(hipGetLastError());
             HIP_TRY(rec(timed ? dc.evpool[evl + 1] : nullptr));
             ev_list.push_back(evl);
@@ -1108,6 +1155,7 @@ struct gm_batch {
     int32_t ng = 0, np = 0;
     int64_t nwaves = 0;
     int stats = 0;
+    int wpe = 2;  // waves per SIMD of the pattern-rule build the graph launches
     void* d_plans = nullptr;
     void* d_gedge = nullptr;
     void* d_sums = nullptr;
@@ -1137,6 +1185,7 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     *out = nullptr;
     if (!g || !plans || np <= 0 || ng < 1) return fail(GM_ERR_INVALID, "gm_batch_create: bad arguments");
     if (ng > 1 && !graph_edge_offsets) return fail(GM_ERR_INVALID, "gm_batch_create: edge offsets required");
+    if (flags & GM_COUNT_DEEP_DONATION) return fail(GM_ERR_UNSUPPORTED, kDeepDonationOff);
     for (int i = 0; i < np; ++i) {
         if (!plans[i]) return fail(GM_ERR_INVALID, "gm_batch_create: null plan");
         if (plans[i]->host.mode != GM_MODE_PATTERN)
@@ -1165,7 +1214,8 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     for (int i = 0; i < np; ++i) nmax = std::max(nmax, nmax_for(plans[i]->host.n));
     int cus = 0, occ = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
-        enum_occupancy<GM_MODE_PATTERN>(nmax, &occ) != hipSuccess)
+        enum_occupancy<GM_MODE_PATTERN>(nmax, b->wpe = enum_wpe(GM_MODE_PATTERN, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0),
+                                        &occ) != hipSuccess)
         return bail(GM_ERR_HIP, "gm_batch_create: device query failed");
     const int blocks = grid_blocks(cus, occ, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0);
     b->nwaves = (int64_t)blocks * kWavesPerBlock;
@@ -1224,6 +1274,7 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         a.queue_cap = kQueueCap;
         a.slice_len = slice_len_of(t);
         a.eager_donation = (flags & GM_COUNT_EAGER_DONATION) ? 1 : 0;
+        a.deep_donation = (flags & GM_COUNT_DEEP_DONATION) ? 1 : 0;
     }
     if (t && t->time_limit_ms > 0) {
         int khz = 0;
@@ -1252,7 +1303,7 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
         hipLaunchKernelGGL(zero_u64_kernel, dim3(64), dim3(256), 0, cap_stream, (unsigned long long*)b->d_sums, ncells);
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, cap_stream, (unsigned long long*)b->d_ctr,
                            (int64_t)(sizeof(Counters) / sizeof(unsigned long long)));
-        enum_launch<GM_MODE_PATTERN>(nmax, blocks, cap_stream, a);
+        enum_launch<GM_MODE_PATTERN>(nmax, b->wpe, blocks, cap_stream, a);
         hipError_t e2 = hipStreamEndCapture(cap_stream, &b->graph);
         e = (e2 != hipSuccess) ? e2 : hipGetLastError();
     }

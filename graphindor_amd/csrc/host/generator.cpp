@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <parallel/algorithm>
 #include <vector>
 
 namespace {
@@ -38,7 +39,9 @@ Graph* build_csr(int32_t n, std::vector<std::pair<int32_t, int32_t> >& pairs) {
         key.push_back(((int64_t)p.second << 32) | (uint32_t)p.first);
     }
     std::vector<std::pair<int32_t, int32_t> >().swap(pairs);
-    std::sort(key.begin(), key.end());
+    // libstdc++'s parallel sort over the OpenMP threads; the keys are
+    // integers, so the result is the same sequence as a serial sort's.
+    __gnu_parallel::sort(key.begin(), key.end());
     key.erase(std::unique(key.begin(), key.end()), key.end());
     Graph* g = new Graph();
     g->v_cnt = n;

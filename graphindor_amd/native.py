@@ -24,6 +24,7 @@ GM_MODE_PATTERN = 0
 GM_MODE_CLIQUE = 1
 GM_COUNT_ALG_BYTES = 1
 GM_COUNT_EAGER_DONATION = 2
+GM_COUNT_DEEP_DONATION = 4
 GM_MAX_PATTERN = 7
 
 
@@ -67,7 +68,8 @@ class GmStats(ctypes.Structure):
                 ("bitmap_block_ms", ctypes.c_double), ("bitmap_block_alg_bytes", ctypes.c_uint64),
                 ("slices", ctypes.c_uint64), ("donations", ctypes.c_uint64), ("queue_retries", ctypes.c_uint64),
                 ("slice_wave_ms", ctypes.c_double), ("idle_wave_ms", ctypes.c_double),
-                ("hub_vertices", ctypes.c_int64)]
+                ("hub_vertices", ctypes.c_int64),
+                ("deep_donations", ctypes.c_uint64)]
 
 
 _gm = None

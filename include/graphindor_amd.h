@@ -95,7 +95,11 @@ typedef struct gm_plan_desc {
 
 /* Launch tuning; zero fields mean "library default". */
 typedef struct gm_tuning {
-    int32_t waves_per_cu;      /* resident waves per CU of the persistent grid */
+    int32_t waves_per_cu;      /* resident waves per CU of the persistent grid (0 = all that fit).
+                                  The pattern rule has two builds: one holding its DFS state in
+                                  registers, 8 waves per CU, and one limited to 128 VGPRs with some
+                                  state spilled to scratch, 16 per CU; 1..8 selects the first, 0 or
+                                  more than 8 the second (gm_count_static always the first) */
     int32_t chunk_edges;       /* most root edges one work-steal claims (claims shrink towards 1 as the
                                   work runs out; 0 = 64) */
     int32_t lds_set_cap;       /* GM_MODE_PATTERN: the last candidate level stages its father sets in
@@ -121,8 +125,9 @@ typedef struct gm_tuning {
                                   harnesses that must not wait on a slow candidate) */
     int32_t hub_degree;        /* GM_MODE_PATTERN: vertices of degree at least this carry their
                                   neighbour set as a bitmap over vertex ids, so that a lookup in a hub's
-                                  list is one load (0 = default 4096, -1 = none; the bitmaps, n/8 bytes
-                                  per hub, at most 4 GiB, are built on the graph at first use) */
+                                  list is one load (0 or -1 = none, the default: whether they pay
+                                  depends on the pattern, so a tuner measures it; the bitmaps, n/8
+                                  bytes per hub, at most 4 GiB, are built on the graph at first use) */
 } gm_tuning;
 
 /* Per-call measurements, filled when the pointer is non-null. */
@@ -137,17 +142,24 @@ typedef struct gm_stats {
     uint64_t bitmap_alg_bytes;/* the bitmap engine's share of alg_bytes */
     double bitmap_block_ms;   /* of bitmap_ms: the block-per-root launches (large roots) */
     uint64_t bitmap_block_alg_bytes; /* of bitmap_alg_bytes: the block-per-root launches */
-    /* Depth-2 slice donation (tuning.slice_len > 0), list engine only: */
+    /* Depth-2 and depth-3 slice donation (tuning.slice_len > 0), list engine only: */
     uint64_t slices;          /* slices pushed to the queue */
     uint64_t donations;       /* donations (each pushes one or more slices) */
     uint64_t queue_retries;   /* queue-head claims lost to another wave */
     double slice_wave_ms;     /* wall time waves spent on donated slices, summed over waves */
     double idle_wave_ms;      /* wall time waves spent waiting for slices, summed over waves */
     int64_t hub_vertices;     /* hub bitmaps the pattern rule used (gm_tuning.hub_degree) */
+    uint64_t deep_donations;  /* of `donations`, those of depth-3 slices: the candidates below one
+                                 depth-2 candidate, donated while waves wait on an empty queue */
 } gm_stats;
 
 /* gm_count flags */
 #define GM_COUNT_ALG_BYTES 1      /* also tally the algorithmic byte model (slower) */
+#define GM_COUNT_DEEP_DONATION 4  /* reserved: depth-3 slice donation (slices of the depth-3 loop
+                                     set below one depth-2 candidate).  Switched off -- gm_count and
+                                     gm_batch_create refuse it with GM_ERR_UNSUPPORTED -- because a
+                                     31-root 5-cycle sample of R-MAT 2^22 that takes 8 s without it
+                                     did not finish in 180 s with it */
 #define GM_COUNT_EAGER_DONATION 2 /* donate depth-2 slices whenever a set is long, not only once the
                                      roots run out (for tests and tuning; counts are unchanged) */
 

@@ -17,7 +17,9 @@ same calls the GPU tests make), counts with the CPU oracle
   rmat23   6-cycle and 3-prism on R-MAT 2^23 ef32 seed 7, likewise
 
 Usage: python tests/golden/make_config_fixtures.py [section ...] [--threads T]
-Sections already in the file are kept unless named.
+Sections already in the file are kept unless named; within a named pattern
+section, patterns already recorded at the same root stride are kept.  The
+3-prism sample on R-MAT 2^23 took 43 minutes of oracle time on 8 threads.
 """
 import argparse
 import hashlib
@@ -42,7 +44,7 @@ PM6 = {"prism": "011100101010110001100011010101001110", "c6": "01000110100001010
 # Root samples: the strides keep each sampled count within minutes of oracle
 # time here and seconds of device time.
 STRIDES22 = {"house": 4096, "c5": 4194304, "k4_pendant": 256}
-STRIDES23 = {"prism": 1048576, "c6": 33554432}
+STRIDES23 = {"prism": 16777216, "c6": 33554432}
 
 
 def clique_adj(k):
@@ -90,14 +92,21 @@ def section_batch(threads, stride=256):
     return {"batch512_s5": rec}
 
 
-def section_pm(threads, key, make, n, patterns, strides, save):
+def section_pm(threads, key, make, n, patterns, strides, save, have=None):
+    """Patterns already in `have` (the section as the file holds it) at the
+    same stride are kept, not counted again."""
     g = make()
     inf = g.info()
     v, e = (x.copy() for x in g.csr())
     csr = O.CSR(v, e)
     rec = {"vertices": int(inf["v_cnt"]), "csr_entries": int(inf["e_cnt"]), "tri_cnt": int(inf["tri_cnt"]),
            "patterns": {}}
+    if have and have.get("csr_entries") == rec["csr_entries"]:
+        rec["patterns"] = {k: x for k, x in have.get("patterns", {}).items() if x.get("stride") == strides.get(k)}
     for name, adj in patterns.items():
+        if name in rec["patterns"]:
+            print(f"{key} {name}: kept from the file", flush=True)
+            continue
         save({key: rec})  # each pattern is kept as soon as it is counted
         s, _ = N.select_schedule(n, adj, v, inf["tri_cnt"], None)
         S = strides[name]
@@ -130,10 +139,11 @@ def main():
         elif sec == "batch":
             data.update(section_batch(a.threads))
         elif sec == "rmat22":
-            data.update(section_pm(a.threads, "rmat22_s4", lambda: N.Graph.rmat(22, 16, 4), 5, PM5, STRIDES22, save))
+            data.update(section_pm(a.threads, "rmat22_s4", lambda: N.Graph.rmat(22, 16, 4), 5, PM5, STRIDES22, save,
+                                   data.get("rmat22_s4")))
         elif sec == "rmat23":
             data.update(section_pm(a.threads, "rmat23_ef32_s7", lambda: N.Graph.rmat(23, 32, 7), 6, PM6, STRIDES23,
-                                   save))
+                                   save, data.get("rmat23_ef32_s7")))
         else:
             raise SystemExit(f"unknown section {sec}")
         save({})
