@@ -230,7 +230,7 @@ PM6 = [("c6", "010001101000010100001010000101100010"), ("prism", "01110010101011
 MODEL_PATH = os.path.join(ROOT, "graphindor_amd", "data", "cost_model.gscm")
 FIXTURES = os.path.join(ROOT, "tests", "golden", "config_counts.json")
 # Root samples e % S == 0 of the pattern legs: those of the parity fixtures.
-PM_STRIDES = {"house": 4096, "c5": 4194304, "k4_pendant": 256, "c6": 33554432, "prism": 16777216}
+PM_STRIDES = {"house": 4096, "c5": 4194304, "k4_pendant": 256, "c6": 134217728, "prism": 16777216}
 # The autotuned pattern-rule knobs: the lane-parallel last level on or off,
 # and hub bitmaps for vertices of degree >= 4096 or none (library default:
 # last level on, no hub bitmaps).

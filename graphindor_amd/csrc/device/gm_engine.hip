@@ -96,12 +96,27 @@ int nmax_for(int n) { return n <= 4 ? 4 : n; }
 const char* const kDeepDonationOff =
     "GM_COUNT_DEEP_DONATION is switched off: depth-3 donation stalled on hub-heavy graphs";
 
-// Waves per SIMD of the build a launch uses: the pattern rule runs the
-// 4-wave build (128 VGPRs, some state in scratch) unless the tuning asks for
-// at most 8 resident waves per CU, which the 2-wave build (all state in
-// registers) holds; the clique rule has the 2-wave build only.
-int enum_wpe(int mode, int req_waves) {
-    return (mode == GM_MODE_PATTERN && (req_waves <= 0 || req_waves > 8)) ? 4 : 2;
+// Waves per SIMD of the build a launch uses.  The clique rule has the
+// 2-wave build only.  The pattern rule runs the 4-wave build (128 VGPRs,
+// some state in scratch) when the tuning asks for more than 8 resident waves
+// per CU, the 2-wave build (all state in registers) when it asks for 8 or
+// fewer; by default the 4-wave build for launches of at least kWideMinRoots
+// root edges.  Small launches keep the 2-wave build: a launch of the 4-wave
+// build costs about 1 ms more (3,072 per-graph 4-motif launches over the
+// 512-graph batch: 9.4 s of kernel time against 6.5 s), and a launch of few
+// roots is bound by its heaviest slices, which the faster waves finish sooner
+// (the 31-root 5-cycle sample: 2.8 s against 3.4 s).
+constexpr int64_t kWideMinRoots = 1 << 14;
+int enum_wpe(int mode, int req_waves, int64_t roots) {
+    if (mode != GM_MODE_PATTERN) return 2;
+    if (req_waves > 0) return req_waves > 8 ? 4 : 2;
+    return roots >= kWideMinRoots ? 4 : 2;
+}
+
+// Root edges e of [lo, hi) with e % stride == 0.
+int64_t sampled_roots(int64_t lo, int64_t hi, int64_t stride) {
+    const int64_t first = (lo + stride - 1) / stride * stride;
+    return first < hi ? (hi - first + stride - 1) / stride : 0;
 }
 
 template <int MODE>
@@ -748,7 +763,10 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
     const int req_waves = (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0;
     // The static workspace runs the 2-wave build: the 4-wave build's spill
     // space is private scratch the runtime would have to provide.
-    const int wpe_pat = fixed_ws ? 2 : enum_wpe(GM_MODE_PATTERN, req_waves);
+    const int64_t roots_est =
+        sampled_roots(graph_edge_offsets ? graph_edge_offsets[0] : 0, graph_edge_offsets ? graph_edge_offsets[ng] : g->nnz,
+                      (t && t->root_stride > 1) ? t->root_stride : 1);
+    const int wpe_pat = fixed_ws ? 2 : enum_wpe(GM_MODE_PATTERN, req_waves, roots_est);
     int blocks_pat = 0, blocks_clq = 0;
     if (nmax_pat) {
         int occ = 0;
@@ -757,7 +775,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
     }
     if (nmax_clq) {
         int occ = 0;
-        HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, enum_wpe(GM_MODE_CLIQUE, req_waves), &occ));
+        HIP_TRY(enum_occupancy<GM_MODE_CLIQUE>(nmax_clq, 2, &occ));
         blocks_clq = grid_blocks(cus, occ, req_waves);
     }
     int64_t nwaves = (int64_t)std::max(blocks_pat, blocks_clq) * kWavesPerBlock;
@@ -1063,7 +1081,7 @@ static int count_impl(const gm_graph* g, const int64_t* graph_edge_offsets, int3
         // bytes, no task -- so the launch, a 9.5 ms scan there, is skipped.
         if (!(use_bitmap && g->max_deg <= bm_max)) {
             HIP_TRY(rec(timed ? dc.evpool[evl] : nullptr));
-            enum_launch<GM_MODE_CLIQUE>(nmax_clq, enum_wpe(GM_MODE_CLIQUE, req_waves), blocks_clq, s, a);
+            enum_launch<GM_MODE_CLIQUE>(nmax_clq, 2, blocks_clq, s, a);
             HIP_TRY(hipGetLastError());
             HIP_TRY(rec(timed ? dc.evpool[evl + 1] : nullptr));
             ev_list.push_back(evl);
@@ -1214,7 +1232,10 @@ int gm_batch_create(const gm_graph* g, const int64_t* graph_edge_offsets, int32_
     for (int i = 0; i < np; ++i) nmax = std::max(nmax, nmax_for(plans[i]->host.n));
     int cus = 0, occ = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device) != hipSuccess ||
-        enum_occupancy<GM_MODE_PATTERN>(nmax, b->wpe = enum_wpe(GM_MODE_PATTERN, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0),
+        enum_occupancy<GM_MODE_PATTERN>(nmax,
+                                        b->wpe = enum_wpe(GM_MODE_PATTERN, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0,
+                                                          sampled_roots(hgedge[0], hgedge[ng],
+                                                                        (t && t->root_stride > 1) ? t->root_stride : 1)),
                                         &occ) != hipSuccess)
         return bail(GM_ERR_HIP, "gm_batch_create: device query failed");
     const int blocks = grid_blocks(cus, occ, (t && t->waves_per_cu > 0) ? t->waves_per_cu : 0);

@@ -98,8 +98,9 @@ typedef struct gm_tuning {
     int32_t waves_per_cu;      /* resident waves per CU of the persistent grid (0 = all that fit).
                                   The pattern rule has two builds: one holding its DFS state in
                                   registers, 8 waves per CU, and one limited to 128 VGPRs with some
-                                  state spilled to scratch, 16 per CU; 1..8 selects the first, 0 or
-                                  more than 8 the second (gm_count_static always the first) */
+                                  state spilled to scratch, 16 per CU; 1..8 selects the first, more
+                                  than 8 the second, 0 the second for launches of at least 16384
+                                  root edges and the first below (gm_count_static always the first) */
     int32_t chunk_edges;       /* most root edges one work-steal claims (claims shrink towards 1 as the
                                   work runs out; 0 = 64) */
     int32_t lds_set_cap;       /* GM_MODE_PATTERN: the last candidate level stages its father sets in

@@ -44,7 +44,7 @@ PM6 = {"prism": "011100101010110001100011010101001110", "c6": "01000110100001010
 # Root samples: the strides keep each sampled count within minutes of oracle
 # time here and seconds of device time.
 STRIDES22 = {"house": 4096, "c5": 4194304, "k4_pendant": 256}
-STRIDES23 = {"prism": 16777216, "c6": 33554432}
+STRIDES23 = {"prism": 16777216, "c6": 134217728}
 
 
 def clique_adj(k):
