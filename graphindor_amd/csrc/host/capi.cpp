@@ -429,8 +429,8 @@ GS_API int gs_costmodel_predict(void* h, const float* x, int32_t rows, float* y)
 // graph `gp` (sample_roots roots, time_limit_ms each); the fastest is kept.
 // info = {candidates, chosen, analytic, learned}, *select_ms = time spent.
 GS_API void* gs_schedule_select_measured(int n, const char* adj, int iep, void* gp, void* model, int top_k,
-                                         int sample_roots, int64_t root_stride, int time_limit_ms, int32_t* info,
-                                         double* select_ms) {
+                                         int sample_roots, int64_t root_stride, int time_limit_ms, int lane_tail,
+                                         int hub_degree, int waves_per_cu, int32_t* info, double* select_ms) {
     void* out = nullptr;
     guarded([&] {
         Pattern p(n, adj);
@@ -439,6 +439,9 @@ GS_API void* gs_schedule_select_measured(int n, const char* adj, int iep, void* 
         opt.sample_roots = sample_roots;
         opt.root_stride = root_stride;
         opt.time_limit_ms = time_limit_ms;
+        opt.tuning.lane_tail = lane_tail;
+        opt.tuning.hub_degree = hub_degree;
+        opt.tuning.waves_per_cu = waves_per_cu;
         ScheduleChoice ch;
         Schedule_IEP* s = select_schedule_measured(p, *(Graph*)gp, (CostModel*)model, iep != 0, opt, &ch);
         out = new GsSchedule{s, FlatPlan(), -1};

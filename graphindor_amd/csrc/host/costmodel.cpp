@@ -261,7 +261,7 @@ Schedule_IEP* select_schedule_measured(const Pattern& pattern, Graph& g, CostMod
     if (pool.size() > 1) {
         gm_graph* dg = g.device_graph();
         const int64_t nnz = g.e_cnt;
-        gm_tuning t{};
+        gm_tuning t = opt.tuning;
         t.root_stride = (int32_t)(opt.root_stride > 0 ? opt.root_stride
                                                       : std::max<int64_t>(1, nnz / std::max(1, opt.sample_roots)));
         t.time_limit_ms = opt.time_limit_ms;

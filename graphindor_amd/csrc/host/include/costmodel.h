@@ -31,6 +31,7 @@
 #include <string>
 #include <vector>
 
+#include "graphindor_amd.h"
 #include "pattern.h"
 #include "schedule_IEP.h"
 
@@ -111,6 +112,10 @@ struct MeasureOptions {
     int sample_roots = 4096;  // roots of the timing sample: edges e % (nnz / sample_roots) == 0
     int64_t root_stride = 0;  // > 0: time on the sample e % root_stride == 0 instead
     int time_limit_ms = 200;
+    // Kernel settings the candidates are timed under (lane_tail, hub_degree,
+    // waves_per_cu, ...: the ones the caller will count with); root_stride
+    // and time_limit_ms are taken from the fields above.
+    gm_tuning tuning{};
 };
 
 // Schedule for `pattern` on graph statistics `gs`: the model's pick among the

@@ -196,8 +196,8 @@ def gs():
             ("gs_costmodel_predict", [VP, VP, ctypes.c_int32, VP], ctypes.c_int),
             ("gs_costmodel_set_margin", [VP, ctypes.c_float], None),
             ("gs_schedule_select_measured", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, VP, ctypes.c_int,
-                                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, VP,
-                                             ctypes.POINTER(ctypes.c_double)], VP),
+                                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, VP, ctypes.POINTER(ctypes.c_double)], VP),
             ("gs_costmodel_margin", [VP], ctypes.c_float),
             ("gs_schedule_select", [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, VP, ctypes.c_int32, ctypes.c_int64, VP,
                                     VP, ctypes.POINTER(ctypes.c_double)], VP),
@@ -494,15 +494,17 @@ def select_schedule(n, adj, vertex, tri_cnt, model=None, iep=True):
 
 
 def select_schedule_measured(n, adj, graph, model=None, iep=True, top_k=3, sample_roots=4096, time_limit_ms=200,
-                             root_stride=0):
+                             root_stride=0, lane_tail=0, hub_degree=0, waves_per_cu=0):
     """(Schedule, info): the model's top_k proposals and the analytic pick
     timed on a root sample of `graph` (a host Graph; its device copy is
     used) -- about sample_roots roots, or exactly the edges e % root_stride
-    == 0 when root_stride > 0 -- the fastest kept."""
+    == 0 when root_stride > 0 -- under the given gm_tuning knobs, the
+    fastest kept."""
     info = (ctypes.c_int32 * 4)()
     ms = ctypes.c_double(0)
     h = gs().gs_schedule_select_measured(n, adj.encode(), 1 if iep else 0, graph.h, model.h if model is not None else None,
-                                         top_k, sample_roots, root_stride, time_limit_ms, info, ctypes.byref(ms))
+                                         top_k, sample_roots, root_stride, time_limit_ms, lane_tail, hub_degree,
+                                         waves_per_cu, info, ctypes.byref(ms))
     s = Schedule(h)
     return s, {"candidates": info[0], "chosen": info[1], "analytic": info[2], "learned": bool(info[3]),
                "select_ms": ms.value}
