@@ -290,6 +290,11 @@ def pm_patterns(N, g, patterns, n, fix, budget_ms=60000, full=(), learned=True):
             assert row["parity"]["oracle_equal"], f"{name}: device differs from the oracle fixture"
 
         def check(r):
+            """r: a raw sum of the analytic schedule on the sample, which
+            every setting of the knobs must reproduce.  (Another schedule
+            assigns the embeddings to other root edges, so its sums over a
+            root sample differ; schedules agree on full counts, which the
+            oracle tests check.)"""
             nonlocal raw
             if r is None:
                 return
@@ -342,9 +347,9 @@ def pm_patterns(N, g, patterns, n, fix, budget_ms=60000, full=(), learned=True):
             log(f"pm {name}: learned selection chose candidate {il['chosen']} in {il['select_ms'] / 1e3:.1f} s")
             pl = N.DevicePlan(sl.plan_desc())
             r3, st3 = timed(pl)
-            check(r3)
             r4, st4 = timed(pl, **best_knob)
-            check(r4)
+            if r3 is not None and r4 is not None:
+                assert r3 == r4, f"{name}: the learned schedule's sum changed with the knobs"
             row.update({"learned_candidate": il["chosen"], "select_ms_learned": round(il["select_ms"], 1),
                         "learned_ms": round(st3.kernel_ms, 3) if st3 else over,
                         "learned_tuned_ms": round(st4.kernel_ms, 3) if st4 else over})
