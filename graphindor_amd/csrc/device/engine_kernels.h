@@ -972,7 +972,11 @@ __device__ __forceinline__ void dfs(Wave& w) {
         const int ln = lane_id();
         int32_t lo = 0, hi = loop.len;
         uint64_t t_loop0 = 0;
-        if constexpr (D == 2 || D == 3) {
+        // Depth-3 slices exist for the pattern rule only; the clique rule
+        // keeps its loops free of their bookkeeping.
+        constexpr bool kSplit = D == 2 || (D == 3 && MODE == GM_MODE_PATTERN);
+        constexpr bool kDeepHost = D == 2 && MODE == GM_MODE_PATTERN;
+        if constexpr (kSplit) {
             // A root's depth-2 and depth-3 candidates may be split: this wave
             // walks [slice_lo, slice_hi) at depth 2 and [s3_lo, s3_hi) at
             // depth 3 (a donated slice, or the whole sets).
@@ -982,7 +986,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
             if (D == 2 ? w.donate : w.donate3) t_loop0 = (uint64_t)uni64((int64_t)wall_clock64());
         }
         for (int32_t base = lo; base < hi; base += kWave) {
-            if constexpr (D == 2 || D == 3) hi = maybe_donate<D>(w, lo, base, hi, t_loop0, 0);
+            if constexpr (kSplit) hi = maybe_donate<D>(w, lo, base, hi, t_loop0, 0);
             int32_t i = base + ln;
             uint32_t cand = 0xffffffffu;
             int64_t rb = 0, re = 0;
@@ -997,7 +1001,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
                 if (v >= bound) return;  // sorted: nothing further qualifies
                 if (out_of_time(w)) return;
                 uint64_t t_b2 = 0;
-                if constexpr (D == 2) {
+                if constexpr (kDeepHost) {
                     // A depth-3 slice rebuilds its one depth-2 candidate's
                     // prefixes, which the donor has charged.
                     w.a.stats = w.deep ? 0 : w.stats0;
@@ -1016,7 +1020,7 @@ __device__ __forceinline__ void dfs(Wave& w) {
                         break;
                     }
                 }
-                if constexpr (D == 2) {
+                if constexpr (kDeepHost) {
                     w.a.stats = w.stats0;
                     if (w.donate3) w.rebuild2_ticks = uni64((int64_t)wall_clock64()) - (int64_t)t_b2;
                     w.cur_i2 = base + j;
