@@ -441,6 +441,7 @@ def intersection_leg(N, dg, v, e):
         out[label] = {"triangles": int(raw), "ms": round(best, 3), "achieved_GBs": round(gbs, 1),
                       "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)}
         log(f"intersection ({label}): {len(e)} pairs, {pair_bytes / 1e9:.2f} GB in {best:.3f} ms = {gbs:.0f} GB/s")
+    assert out["bitmap_engine"]["triangles"] == out["list_engine"]["triangles"], "the two engines count differently"
     return out
 
 
